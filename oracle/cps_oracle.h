@@ -1,0 +1,118 @@
+/*
+ * cps_oracle.h -- CPU restatement of CPS's even-odd preconditioned DWF / Mobius
+ * operator (MatPcDagMatPc) and the DiracOp::InvCgShift solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the parity checker for the HIP
+ * library in cps_veloc_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path never calls it.
+ *
+ * All arithmetic is FP64.  Every layout is CPS's own:
+ *   gauge   (WILSON order)  u[(cbsite + vh*parity)*72 + mu*18 + re + 2*(row + 3*col)]
+ *           even sites first; element (row,col) is canonical U(row,col)
+ *           (cps_pp/src/util/lattice/convert/convert_func.C:372-374,403-405).
+ *   spinor  [spin 4][colour 3][re,im] = 24 reals
+ *           (cps_pp/src/util/dirac_op/d_op_wilson/noarch/wilson_dslash.C:60).
+ *   cb site (x/2) + (lx/2)*(y + ly*(z + lz*t))          (wilson_dslash.C:288).
+ *   DWF half vector    [s][cbsite][24], slice s holds 4D parity (s+cb)%2
+ *                      (d_op_dwf/noarch/dwf_dslash_4.C:107-137).
+ *   Mobius half vector [s][cbsite][24], every slice holds the same 4D parity
+ *                      (d_op_zmobius/noarch/zmobius_dslash_4.C:109-121).
+ *
+ * Parity pinning of each function is listed in oracle/README.md.
+ */
+#ifndef CPS_ORACLE_H
+#define CPS_ORACLE_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Lattice / action description shared by all oracle calls. */
+typedef struct {
+  int L[4];          /* local 4D extents lx,ly,lz,lt (all even)          */
+  int ls;            /* fifth-dimension extent                           */
+  double m5;         /* domain-wall height M5  (GJP.DwfHeight)           */
+  double a5inv;      /* 1/a5                   (GJP.DwfA5Inv)            */
+  double mob_b;      /* Mobius b (GJP.Mobius_b); unused for DWF          */
+  double mob_c;      /* Mobius c (GJP.Mobius_c); unused for DWF          */
+  int action;        /* 0 = DWF (DiracOpDwf), 1 = Mobius (DiracOpMobius) */
+} orc_params;
+
+/* ---- layouts / boundary conditions (a13) ---------------------------- */
+/* CANONICAL gauge [site][mu][row][col][re,im] -> WILSON order, applying
+ * the anti-periodic sign of Lattice::BondCond for every mu with bc[mu]!=0. */
+void orc_gauge_canonical_to_wilson(double *u_wilson, const double *u_canon,
+                                   const int L[4], const int bc[4]);
+/* One CANONICAL 4D fermion slice [site][24] -> half-lattice of parity p. */
+void orc_fermion_extract_cb(double *half, const double *canon, const int L[4], int p);
+void orc_fermion_insert_cb(double *canon, const double *half, const int L[4], int p);
+
+/* ---- a1: Wilson 4D hopping term ------------------------------------- */
+void orc_wilson_dslash(double *chi, const double *u, const double *psi,
+                       int cb, int dag, const int L[4]);
+
+/* ---- a2..a4: DWF ----------------------------------------------------- */
+void orc_dwf_dslash_5_plus(double *out, const double *in, double mass, int dag,
+                           const orc_params *p);
+void orc_dwf_dslash(double *out, const double *u, const double *in, double mass,
+                    int cb, int dag, const orc_params *p);
+void orc_dwf_m(double *out, const double *u, const double *in, double mass,
+               const orc_params *p, double *tmp2);
+void orc_dwf_mdag(double *out, const double *u, const double *in, double mass,
+                  const orc_params *p, double *tmp2);
+void orc_dwf_mdagm(double *out, const double *u, const double *in, double *dot_prd,
+                   double mass, const orc_params *p, double *tmp1, double *tmp2);
+
+/* ---- a6..a9: Mobius (zmobius arithmetic, complex per-s coefficients) --- */
+/* coefficient arrays are complex [ls][2]                                   */
+void orc_zmobius_kappa_dslash_5_plus(double *out, const double *in, double mass,
+                                     int dag, const double *fact, const orc_params *p);
+void orc_zmobius_dslash_4(double *out, const double *u, const double *in, int cb,
+                          int dag, double mass, const double *zb, const double *zc,
+                          const orc_params *p, double *tmp3);
+void orc_zmobius_m5inv(double *inout, double mass, int dag, const double *K,
+                       const orc_params *p);
+void orc_mobius_m(double *out, const double *u, const double *in, double mass,
+                  const orc_params *p, double *tmp2, double *tmp3);
+void orc_mobius_mdag(double *out, const double *u, const double *in, double mass,
+                     const orc_params *p, double *tmp2, double *tmp3);
+void orc_mobius_mdagm(double *out, const double *u, const double *in, double *dot_prd,
+                      double mass, const orc_params *p, double *tmp1, double *tmp2,
+                      double *tmp3);
+/* Mobius per-s coefficient arrays as DiracOpMobius builds them
+ * (d_op_mobius.C:105-125): zb=b, zc=c, kappa_b, kappa_ratio=kappa_b/kappa_c. */
+void orc_mobius_coefficients(const orc_params *p, double *zb, double *zc,
+                             double *zkappa_b, double *zkappa_ratio);
+
+/* ---- a5/a10: DiracOp front ends --------------------------------------- */
+/* Number of reals in one CG (half) vector: 24*V4*ls/2. */
+size_t orc_half_size(const orc_params *p);
+void orc_matpcdagmatpc(double *out, const double *u, const double *in, double *dot_prd,
+                       double mass, const orc_params *p);
+void orc_matpc(double *out, const double *u, const double *in, double mass,
+               int dag, const orc_params *p);
+/* DWF:    out (5D parity 1-cb) = dwf_dslash(in, cb, dag)
+ * Mobius: out (4D parity 1-cb) = zmobius_dslash_4(in, cb, dag)            */
+void orc_dslash(double *out, const double *u, const double *in, double mass,
+                int cb, int dag, const orc_params *p);
+
+/* ---- a11: DiracOp::InvCgShift (shift = NULL) --------------------------- */
+/* Returns CPS's count itr+1.  hist (may be NULL, length >= max_iter+1)
+ * receives |res|^2 after each iteration, hist[0] = initial |res|^2.       */
+int orc_invcg(double *sol, const double *src, const double *u, double mass,
+              double stop_rsd, int max_iter, double *true_res, double *hist,
+              const orc_params *p);
+
+/* ---- a12: BLAS-1 ------------------------------------------------------- */
+double orc_norm_sq(const double *a, size_t n);   /* Vector::NormSqNode, serial */
+
+/* ---- (f1): DiracOpDwf::MatInv / DiracOpMobius::MatInv ----------------- */
+/* full vectors: DWF [5D-odd | 5D-even], Mobius [4D-even | 4D-odd]        */
+int orc_matinv(double *out_full, double *in_full, const double *u, double mass,
+               double stop_rsd, int max_iter, double *true_res, const orc_params *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
