@@ -1,0 +1,250 @@
+"""ctypes binding of libcps_dwf_gpu.so with the DiracOpDwf / DiracOpMobius
+call surface of CPS (cps_pp/include/util/dirac_op.h):
+
+    op = DiracOpDwfGpu(gauge, dims, ls, m5=1.8)       # DiracOpDwf ctor
+    dot = op.MatPcDagMatPc(out, inp, mass)             # dirac_op.h:259
+    op.MatPc(out, inp, mass); op.MatPcDag(out, inp, mass)
+    op.Dslash(out, inp, mass, cb, dag)
+    iters, true_res = op.InvCg(out, inp, CgArg(mass, ...))   # dirac_op.h:138
+    iters, true_res = op.MatInv(out, inp, CgArg(...))
+
+Vectors are float64 numpy arrays in CPS half-vector order.  The HIP library
+must be built (cps_veloc_amd/build.py); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcps_dwf_gpu.so")
+
+OK, BREAKDOWN, MAXITER, BAD_ARG, HIP_ERROR = 0, 1, 2, 3, 4
+DWF, MOBIUS = 0, 1
+F64, F32 = 0, 1
+_PREC = {"f64": F64, "f32": F32}
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_lib = None
+
+# every entry point of include/dwf_gpu.h
+EXPORTS = (
+    "dwfgpu_create", "dwfgpu_destroy", "dwfgpu_half_size", "dwfgpu_last_error",
+    "dwfgpu_matpcdagmatpc", "dwfgpu_matpc", "dwfgpu_dslash", "dwfgpu_invcg",
+    "dwfgpu_matinv", "dwfgpu_vec_create", "dwfgpu_vec_destroy", "dwfgpu_vec_upload",
+    "dwfgpu_vec_download", "dwfgpu_matpcdagmatpc_dev", "dwfgpu_invcg_dev",
+    "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost",
+)
+
+
+class DwfGpuError(RuntimeError):
+    def __init__(self, code, where):
+        msg = lib().dwfgpu_last_error().decode(errors="replace")
+        super().__init__(f"{where} failed with status {code}: {msg}")
+        self.code = code
+
+
+class CgArgC(C.Structure):
+    _fields_ = [("mass", C.c_double), ("max_num_iter", C.c_int),
+                ("stop_rsd", C.c_double), ("true_rsd", C.c_double)]
+
+
+@dataclass
+class CgArg:
+    """Mirror of CPS's CgArg (cps_pp/include/input/cg_arg.x:3-28)."""
+    mass: float
+    max_num_iter: int = 10000
+    stop_rsd: float = 1e-10
+    true_rsd: float = 1e-10
+
+    def c(self):
+        return CgArgC(self.mass, self.max_num_iter, self.stop_rsd, self.true_rsd)
+
+
+def lib():
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run cps_veloc_amd/build.py "
+                               "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        H = C.c_void_p
+        L.dwfgpu_last_error.restype = C.c_char_p
+        L.dwfgpu_create.argtypes = [_dp, C.c_int * 4, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_int * 4, C.c_int,
+                                    C.POINTER(H)]
+        L.dwfgpu_destroy.argtypes = [H]
+        L.dwfgpu_half_size.argtypes = [H]
+        L.dwfgpu_half_size.restype = C.c_size_t
+        L.dwfgpu_matpcdagmatpc.argtypes = [H, C.c_double, _dp, _dp, C.POINTER(C.c_double)]
+        L.dwfgpu_matpc.argtypes = [H, C.c_double, _dp, _dp, C.c_int]
+        L.dwfgpu_dslash.argtypes = [H, C.c_double, _dp, _dp, C.c_int, C.c_int]
+        L.dwfgpu_invcg.argtypes = [H, C.POINTER(CgArgC), _dp, _dp, C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int)]
+        L.dwfgpu_matinv.argtypes = [H, C.POINTER(CgArgC), _dp, _dp, C.POINTER(C.c_double),
+                                    C.POINTER(C.c_int)]
+        L.dwfgpu_vec_create.argtypes = [H, C.POINTER(H)]
+        L.dwfgpu_vec_destroy.argtypes = [H]
+        L.dwfgpu_vec_upload.argtypes = [H, _dp]
+        L.dwfgpu_vec_download.argtypes = [H, _dp]
+        L.dwfgpu_matpcdagmatpc_dev.argtypes = [H, C.c_double, H, H, C.POINTER(C.c_double)]
+        L.dwfgpu_invcg_dev.argtypes = [H, C.POINTER(CgArgC), H, H, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_int)]
+        L.dwfgpu_time_matpcdagmatpc.argtypes = [H, C.c_double, H, H, C.c_int,
+                                                C.POINTER(C.c_double), _dp, _dp,
+                                                C.POINTER(C.c_int)]
+        L.dwfgpu_matpcdagmatpc_cost.argtypes = [H, C.POINTER(C.c_double),
+                                                C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _chk(code, where, allow=(OK,)):
+    if code not in allow:
+        raise DwfGpuError(code, where)
+    return code
+
+
+def _arr(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+class DeviceVector:
+    """A CG half vector resident in HBM (dwfgpu_vec_t)."""
+
+    def __init__(self, op: "_DiracOpGpu", host=None):
+        self.op = op
+        self.h = C.c_void_p()
+        _chk(lib().dwfgpu_vec_create(op.h, C.byref(self.h)), "dwfgpu_vec_create")
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host):
+        _chk(lib().dwfgpu_vec_upload(self.h, _arr(host)), "dwfgpu_vec_upload")
+
+    def download(self):
+        out = np.empty(self.op.n)
+        _chk(lib().dwfgpu_vec_download(self.h, out), "dwfgpu_vec_download")
+        return out
+
+    def free(self):
+        if self.h:
+            lib().dwfgpu_vec_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class _DiracOpGpu:
+    action = DWF
+
+    def __init__(self, gauge_canonical, dims, ls, m5=1.8, a5inv=1.0, b=1.0, c=0.0,
+                 bc=(0, 0, 0, 1), precision="f64"):
+        g = np.ascontiguousarray(gauge_canonical)
+        g = g.view(np.float64).reshape(-1) if np.iscomplexobj(g) else g.reshape(-1)
+        self.dims = tuple(int(d) for d in dims)
+        self.ls = int(ls)
+        self.h = C.c_void_p()
+        _chk(lib().dwfgpu_create(_arr(g), (C.c_int * 4)(*self.dims), self.ls, self.action,
+                                 m5, a5inv, b, c, (C.c_int * 4)(*bc), _PREC[precision],
+                                 C.byref(self.h)), "dwfgpu_create")
+        self.n = int(lib().dwfgpu_half_size(self.h))
+
+    def close(self):
+        if self.h:
+            lib().dwfgpu_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- DiracOp surface on host arrays -----------------------------------
+    def MatPcDagMatPc(self, out, inp, mass):
+        d = C.c_double()
+        _chk(lib().dwfgpu_matpcdagmatpc(self.h, mass, out, _arr(inp), C.byref(d)),
+             "MatPcDagMatPc")
+        return d.value
+
+    def MatPc(self, out, inp, mass):
+        _chk(lib().dwfgpu_matpc(self.h, mass, out, _arr(inp), 0), "MatPc")
+
+    def MatPcDag(self, out, inp, mass):
+        _chk(lib().dwfgpu_matpc(self.h, mass, out, _arr(inp), 1), "MatPcDag")
+
+    def Dslash(self, out, inp, mass, cb, dag=0):
+        _chk(lib().dwfgpu_dslash(self.h, mass, out, _arr(inp), cb, dag), "Dslash")
+
+    def InvCg(self, out, inp, cg_arg: CgArg):
+        """out: initial guess in, solution out.  Returns (iters, true_res, status)."""
+        tr, it = C.c_double(), C.c_int()
+        a = cg_arg.c()
+        st = _chk(lib().dwfgpu_invcg(self.h, C.byref(a), out, _arr(inp), C.byref(tr),
+                                     C.byref(it)), "InvCg", allow=(OK, MAXITER))
+        return it.value, tr.value, st
+
+    def MatInv(self, out, inp, cg_arg: CgArg):
+        tr, it = C.c_double(), C.c_int()
+        a = cg_arg.c()
+        st = _chk(lib().dwfgpu_matinv(self.h, C.byref(a), out, _arr(inp), C.byref(tr),
+                                      C.byref(it)), "MatInv", allow=(OK, MAXITER))
+        return it.value, tr.value, st
+
+    # --- device-resident fast path ----------------------------------------
+    def vector(self, host=None):
+        return DeviceVector(self, host)
+
+    def MatPcDagMatPc_dev(self, out: DeviceVector, inp: DeviceVector, mass):
+        d = C.c_double()
+        _chk(lib().dwfgpu_matpcdagmatpc_dev(self.h, mass, out.h, inp.h, C.byref(d)),
+             "MatPcDagMatPc_dev")
+        return d.value
+
+    def InvCg_dev(self, sol: DeviceVector, src: DeviceVector, cg_arg: CgArg):
+        tr, it = C.c_double(), C.c_int()
+        a = cg_arg.c()
+        st = _chk(lib().dwfgpu_invcg_dev(self.h, C.byref(a), sol.h, src.h, C.byref(tr),
+                                         C.byref(it)), "InvCg_dev", allow=(OK, MAXITER))
+        return it.value, tr.value, st
+
+    def time_MatPcDagMatPc(self, out: DeviceVector, inp: DeviceVector, mass, reps):
+        ms = C.c_double()
+        msk = np.zeros(8)
+        byk = np.zeros(8)
+        nk = C.c_int()
+        _chk(lib().dwfgpu_time_matpcdagmatpc(self.h, mass, out.h, inp.h, reps, C.byref(ms),
+                                             msk, byk, C.byref(nk)), "time_MatPcDagMatPc")
+        return ms.value, msk[:nk.value], byk[:nk.value]
+
+    def cost(self):
+        b, f = C.c_double(), C.c_double()
+        _chk(lib().dwfgpu_matpcdagmatpc_cost(self.h, C.byref(b), C.byref(f)), "cost")
+        return b.value, f.value
+
+
+class DiracOpDwfGpu(_DiracOpGpu):
+    """DiracOpDwf (cps_pp/src/util/dirac_op/d_op_dwf/d_op_dwf.C)."""
+    action = DWF
+
+    def __init__(self, gauge_canonical, dims, ls, m5=1.8, a5inv=1.0, bc=(0, 0, 0, 1),
+                 precision="f64"):
+        super().__init__(gauge_canonical, dims, ls, m5, a5inv, 1.0, 0.0, bc, precision)
+
+
+class DiracOpMobiusGpu(_DiracOpGpu):
+    """DiracOpMobius (cps_pp/src/util/dirac_op/d_op_mobius/d_op_mobius.C)."""
+    action = MOBIUS
+
+    def __init__(self, gauge_canonical, dims, ls, m5=1.8, b=1.5, c=0.5, a5inv=1.0,
+                 bc=(0, 0, 0, 1), precision="f64"):
+        super().__init__(gauge_canonical, dims, ls, m5, a5inv, b, c, bc, precision)

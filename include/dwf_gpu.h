@@ -1,0 +1,122 @@
+/*
+ * dwf_gpu.h -- C ABI of the MI355X (gfx950) domain-wall / Mobius operator and
+ * CG solver.  Every entry point replaces one piece of CPS's DiracOp call
+ * surface (paths relative to cps_pp/); plain pointers and sizes only.
+ *
+ * Vectors passed as host arrays are FP64 CPS half vectors, 24*V4*Ls/2 reals,
+ * in CPS's own order [s][checkerboard site][spin][colour][re,im]:
+ *   DWF    : the 5D-odd half, slice s holds 4D parity (s+1)%2
+ *            (src/util/lattice/convert/convert.C:704-760)
+ *   Mobius : the 4D-odd half (every slice 4D-odd)
+ *            (src/util/lattice/convert/convert.C:797-831).
+ * Full vectors (MatInv) are [5D-odd | 5D-even] for DWF and [4D-even | 4D-odd]
+ * for Mobius, as CPS stores them.
+ *
+ * Handles are not thread-safe; every call is synchronous with respect to the
+ * host and runs on the handle's own HIP stream.
+ */
+#ifndef DWF_GPU_H
+#define DWF_GPU_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (CPS aborts through ERR.General instead) */
+#define DWFGPU_OK 0
+#define DWFGPU_BREAKDOWN 1 /* CG found <p, A p> == 0 (CPS: exit(5), inv_cg.C:326-331) */
+#define DWFGPU_MAXITER 2   /* CG hit max_num_iter (CPS: VRB.Warn, inv_cg.C:392-396)  */
+#define DWFGPU_BAD_ARG 3
+#define DWFGPU_HIP_ERROR 4
+
+/* actions */
+#define DWFGPU_DWF 0    /* DiracOpDwf    (src/util/dirac_op/d_op_dwf/d_op_dwf.C)       */
+#define DWFGPU_MOBIUS 1 /* DiracOpMobius (src/util/dirac_op/d_op_mobius/d_op_mobius.C) */
+
+/* storage precision of the operator's fields on the device */
+#define DWFGPU_F64 0
+#define DWFGPU_F32 1
+
+typedef struct dwfgpu_s *dwfgpu_t;
+typedef struct dwfgpu_vec_s *dwfgpu_vec_t;
+
+/* Mirrors CgArg (include/input/cg_arg.x:3-28): the fields InvCg reads. */
+typedef struct {
+  double mass;
+  int max_num_iter;
+  double stop_rsd;
+  double true_rsd; /* unused by InvCg, kept for layout parity with CgArg */
+} dwfgpu_cg_arg;
+
+/* Construct the operator: the DiracOp ctor's BondCond + Convert(WILSON) +
+ * dwf_init / DiracOpMobius coefficient set-up
+ * (d_op_base/comsrc/dirac_op_base.C:95-170, d_op_dwf/d_op_dwf.C:67-123,
+ *  d_op_mobius/d_op_mobius.C:57-140).
+ * gauge_canonical: 4*V4*18 FP64 reals, CANONICAL order
+ *   [site x+lx*(y+ly*(z+lz*t))][mu][row][col][re,im] (include/util/lattice.h:327).
+ * bc[mu] = 1 multiplies the links U_mu(x) with x_mu = L_mu-1 by -1
+ *   (Lattice::BondCond, src/util/lattice/lattice_base/lattice_base.C:5336-5391).
+ * mob_b, mob_c are ignored for DWF.  precision: DWFGPU_F64 or DWFGPU_F32.   */
+int dwfgpu_create(const double *gauge_canonical, const int dims[4], int ls, int action,
+                  double m5, double a5inv, double mob_b, double mob_c, const int bc[4],
+                  int precision, dwfgpu_t *out);
+int dwfgpu_destroy(dwfgpu_t h);
+/* number of reals in one half vector: 24*V4*Ls/2 */
+size_t dwfgpu_half_size(dwfgpu_t h);
+/* message describing the last failure on any handle of this thread */
+const char *dwfgpu_last_error(void);
+
+/* ---- host-array entry points (CPS half-vector order) ------------------ */
+/* DiracOp::MatPcDagMatPc(out, in, dot_prd)       include/util/dirac_op.h:259
+ *   (d_op_dwf.C:174-200 -> dwf_mdagm, d_op_mobius.C:200-218 -> mobius_mdagm)
+ * dot_prd (may be NULL) receives |MatPc in|^2.                              */
+int dwfgpu_matpcdagmatpc(dwfgpu_t h, double mass, double *out, const double *in,
+                         double *dot_prd);
+/* DiracOp::MatPc / MatPcDag  (d_op_dwf.C:245-300, d_op_mobius.C:320-372) */
+int dwfgpu_matpc(dwfgpu_t h, double mass, double *out, const double *in, int dag);
+/* DiracOp::Dslash(out, in, cb, dag)  (d_op_dwf.C:210-236 -> dwf_dslash;
+ * d_op_mobius.C:275-291 -> mobius_dslash_4 = zmobius_dslash_4).
+ * in: half of parity cb, out: half of parity 1-cb.                          */
+int dwfgpu_dslash(dwfgpu_t h, double mass, double *out, const double *in, int cb,
+                  int dag);
+/* DiracOp::InvCg(out, in, true_res) (include/util/dirac_op.h:138) through
+ * InvCgShift(out, in, 0, true_res, NULL) (d_op_base/comsrc/inv_cg.C:84-448):
+ * sol holds the initial guess on entry.  *iters receives CPS's count
+ * (loop iterations + 1, inv_cg.C:446).  Returns DWFGPU_MAXITER when the
+ * iteration cap was hit (solution and true_res still filled).               */
+int dwfgpu_invcg(dwfgpu_t h, const dwfgpu_cg_arg *arg, double *sol, const double *src,
+                 double *true_res, int *iters);
+/* DiracOpDwf::MatInv (d_op_dwf.C:316-507) / DiracOpMobius::MatInv
+ * (d_op_mobius.C:388-696, ORIG pc) on full vectors; in is preserved
+ * (PRESERVE_YES), out holds the initial guess of its CG half on entry.      */
+int dwfgpu_matinv(dwfgpu_t h, const dwfgpu_cg_arg *arg, double *out, const double *in,
+                  double *true_res, int *iters);
+
+/* ---- device-resident vectors (no PCIe traffic per call) --------------- */
+int dwfgpu_vec_create(dwfgpu_t h, dwfgpu_vec_t *v);
+int dwfgpu_vec_destroy(dwfgpu_vec_t v);
+int dwfgpu_vec_upload(dwfgpu_vec_t v, const double *host_cps_order);
+int dwfgpu_vec_download(dwfgpu_vec_t v, double *host_cps_order);
+int dwfgpu_matpcdagmatpc_dev(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_vec_t in,
+                             double *dot_prd);
+int dwfgpu_invcg_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol,
+                     dwfgpu_vec_t src, double *true_res, int *iters);
+
+/* ---- instrumentation (the role of CPS's print_flops / CGflops) --------- */
+/* Times `reps` MatPcDagMatPc applications on the handle's stream with HIP
+ * events around each of its kernel launches.  ms_app: mean per application;
+ * ms_kernel[k] (k < *nkernel on return, at most 8): mean per launch of the
+ * k-th kernel of one application; bytes_kernel[k]: its algorithmic HBM bytes
+ * (inputs read once, outputs written once).                                  */
+int dwfgpu_time_matpcdagmatpc(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_vec_t in,
+                              int reps, double *ms_app, double *ms_kernel,
+                              double *bytes_kernel, int *nkernel);
+/* Algorithmic bytes and flops of one MatPcDagMatPc application with the
+ * handle's storage precision (see include/roofline.h).                      */
+int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, double *bytes, double *flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
