@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Benchmark of the even-odd preconditioned Mobius MatPcDagMatPc and CG.
+
+Default workload: 24^3x64, Ls=12 Mobius (b=1.5, c=0.5, M5=1.8, mf=0.005),
+links reunitarize(exp(i*0.4*H)), seeded unit Gaussian source, FP64.
+One step = one MatPcDagMatPc application on HBM-resident vectors; the value
+is its algorithmic HBM throughput (include/roofline.h byte model).  The line
+also carries the time to a 1e-10 CG solve, the dominant kernel's roofline and
+the CPU oracle timed on the host cores.
+
+Prints exactly one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Time-to-1e-10 solve vs plain FP64 CG (same GPU); MatPcDagMatPc HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    "mobius_24x24x24x64_ls12": dict(action="mobius", dims=(24, 24, 24, 64), ls=12, mass=0.005,
+                                    eps=0.4, b=1.5, c=0.5),
+    "mobius_16x16x16x32_ls16": dict(action="mobius", dims=(16, 16, 16, 32), ls=16, mass=0.01,
+                                    eps=0.3, b=1.5, c=0.5),
+    "dwf_16x16x16x32_ls16": dict(action="dwf", dims=(16, 16, 16, 32), ls=16, mass=0.01, eps=0.3),
+    "dwf_8x8x8x8_ls8": dict(action="dwf", dims=(8, 8, 8, 8), ls=8, mass=0.04, eps=0.3),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profile(workload: str, precision: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprof
+    PMC summary (profiles/<workload>_<precision>_traffic.json), if present."""
+    path = os.path.join(ROOT, "profiles", f"{workload}_{precision}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_baseline(wl, U, src, seconds_hint=True):
+    """The oracle (C restatement of CPS's noarch path) timed on the host."""
+    from oracle.oracle import Oracle
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(wl["dims"], wl["ls"], wl["action"], m5=1.8, b=wl.get("b", 1.5),
+               c=wl.get("c", 0.5), gauge_canonical=U)
+    o.matpcdagmatpc(src, wl["mass"])  # warm (page faults, OpenMP start)
+    t0 = time.perf_counter()
+    o.matpcdagmatpc(src, wl["mass"])
+    dt = time.perf_counter() - t0
+    return dt, cores
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="mobius_24x24x24x64_ls12", choices=sorted(WORKLOADS))
+    ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-iter", type=int, default=20000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU "
+              "sharding of this operator is a design note, not a built path.", file=sys.stderr)
+        sys.exit(2)
+
+    from cps_veloc_amd import lattice as lat
+    from cps_veloc_amd.dwf import CgArg, DiracOpDwfGpu, DiracOpMobiusGpu
+
+    wl = WORKLOADS[args.workload]
+    dims, ls, mass = wl["dims"], wl["ls"], wl["mass"]
+    vh = int(np.prod(dims)) // 2
+    log(f"[bench] workload {args.workload} precision {args.precision}: generating inputs")
+    t0 = time.perf_counter()
+    U = lat.random_gauge(dims, wl["eps"], 1234)
+    src = lat.to_reals(lat.gaussian_spinor((ls, vh), 5678))
+    log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s; uploading")
+    if wl["action"] == "dwf":
+        op = DiracOpDwfGpu(U, dims, ls, m5=1.8, precision=args.precision)
+    else:
+        op = DiracOpMobiusGpu(U, dims, ls, m5=1.8, b=wl["b"], c=wl["c"],
+                              precision=args.precision)
+    vin, vout = op.vector(src), op.vector()
+    bytes_app, flops_app = op.cost()
+
+    for _ in range(args.warmup):
+        op.MatPcDagMatPc_dev(vout, vin, mass)
+    log(f"[bench] warmup done ({args.warmup}); timing {args.steps} applications")
+    tw = time.perf_counter()
+    ms_app, ms_k, bytes_k = op.time_MatPcDagMatPc(vout, vin, mass, args.steps)
+    wall = time.perf_counter() - tw
+    gbs = bytes_app / (ms_app * 1e-3) / 1e9
+    gflops = flops_app / (ms_app * 1e-3) / 1e9
+    log(f"[bench] MatPcDagMatPc {ms_app:.3f} ms  {gbs:.0f} GB/s  {gflops:.0f} GFLOP/s "
+        f"(wall {wall:.2f}s); kernels ms {np.round(ms_k, 4).tolist()}")
+    hop = int(np.argmax(ms_k[:4]))
+    k_gbs = bytes_k[hop] / (ms_k[hop] * 1e-3) / 1e9
+    copy_gbs = op.time_copy(2 * 1024 ** 3, 10)
+    log(f"[bench] dominant kernel #{hop}: {k_gbs:.0f} GB/s; copy kernel {copy_gbs:.0f} GB/s")
+    prof = traffic_from_profile(args.workload, args.precision)
+
+    solve = None
+    if not args.no_solve:
+        vsol = op.vector()
+        t1 = time.perf_counter()
+        it, tr, st = op.InvCg_dev(vsol, vin, CgArg(mass, args.max_iter, 1e-10))
+        ts = time.perf_counter() - t1
+        solve = {"plain_fp64_cg": {"seconds": round(ts, 4), "iters": it,
+                                   "true_res": tr, "status": st,
+                                   "stop_rsd": 1e-10}}
+        log(f"[bench] plain CG: {it} iters, true res {tr:.3e}, {ts:.2f}s")
+        vsol.free()
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU oracle: one MatPcDagMatPc application")
+        dt, cores = cpu_baseline(wl, U, src)
+        cpu = {"value": round(bytes_app / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
+               "kind": "port",
+               "sample": f"1 MatPcDagMatPc application of the same workload in "
+                         f"{dt:.2f}s (oracle/cps_oracle.c, OpenMP over sites, FP64)"}
+        log(f"[bench] CPU oracle {dt:.2f}s on {cores} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_app, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "f64" else "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "lattice": list(dims), "ls": ls,
+                   "action": wl["action"], "mass": mass, "m5": 1.8,
+                   "mobius_b": wl.get("b"), "mobius_c": wl.get("c"), "gauge_eps": wl["eps"],
+                   "precision": args.precision},
+        "gflops": round(gflops, 2),
+        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        "copy_kernel_gbs": round(copy_gbs, 1),
+        "kernels_ms": [round(float(x), 5) for x in ms_k],
+        "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": None if prof is None else prof.get("traffic_bytes"),
+                     "kernel": ["hop_m5inv", "hop_schur", "hop_m5inv_dag",
+                                "hop_schur_dag"][hop],
+                     "algorithmic_bytes": bytes_k[hop]},
+        "cpu_baseline": cpu,
+        "solve": solve,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -8,10 +8,11 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcps_dwf_gpu.so")
-SOURCES = [os.path.join(HERE, "csrc", "dwf_gpu.hip")]
+SOURCES = [os.path.join(HERE, "csrc", "dwf_gpu.hip"), os.path.join(HERE, "csrc", "synth.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "dwf_kernels.hpp"),
            os.path.join(ROOT, "include", "dwf_gpu.h"),
-           os.path.join(ROOT, "include", "roofline.h")]
+           os.path.join(ROOT, "include", "roofline.h"),
+           os.path.join(ROOT, "include", "dwf_synth.h")]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -776,6 +776,42 @@ int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, double* bytes, double* flops) {
   return DWFGPU_OK;
 }
 
+int dwfgpu_time_copy(dwfgpu_t h, size_t bytes, int reps, double* gbs) {
+  if (!h || reps < 1 || bytes < 16 || !gbs) return DWFGPU_BAD_ARG;
+  const size_t n16 = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, n16 * 16));
+  if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+    (void)hipFree(a);
+    set_err("dwfgpu_time_copy: hipMalloc failed");
+    return DWFGPU_HIP_ERROR;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = DWFGPU_OK;
+  if (hipMemsetAsync(a, 1, n16 * 16, h->st) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+      hipEventCreate(&e1) != hipSuccess)
+    st = DWFGPU_HIP_ERROR;
+  const dim3 grid(8192), block(256);
+  if (!st) {  // one untimed pass to fault in the pages
+    hipLaunchKernelGGL(dwfk::k_copy16, grid, block, 0, h->st, (uint4*)b, (const uint4*)a, n16);
+    if (hipEventRecord(e0, h->st) != hipSuccess) st = DWFGPU_HIP_ERROR;
+  }
+  for (int r = 0; r < reps && !st; ++r)
+    hipLaunchKernelGGL(dwfk::k_copy16, grid, block, 0, h->st, (uint4*)b, (const uint4*)a, n16);
+  if (!st && (hipGetLastError() != hipSuccess || hipEventRecord(e1, h->st) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess))
+    st = DWFGPU_HIP_ERROR;
+  float ms = 0;
+  if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = DWFGPU_HIP_ERROR;
+  if (!st) *gbs = 2.0 * n16 * 16 * reps / (ms * 1e-3) / 1e9;
+  if (st) set_err("dwfgpu_time_copy: HIP failure");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return st;
+}
+
 int dwfgpu_time_matpcdagmatpc(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_vec_t in,
                               int reps, double* ms_app, double* ms_kernel, double* bytes_kernel,
                               int* nkernel) {

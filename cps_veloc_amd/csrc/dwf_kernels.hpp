@@ -557,6 +557,14 @@ __global__ void __launch_bounds__(BLAS_THREADS) k_scale(R* __restrict__ x, doubl
     x[k] = ar * x[k];
 }
 
+// reference streaming copy (16-byte words)
+__global__ void __launch_bounds__(256)
+k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n16;
+       k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = src[k];
+}
+
 // deterministic final sum of per-block partials (one work-group)
 __global__ void __launch_bounds__(256)
 k_reduce_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {

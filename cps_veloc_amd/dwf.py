@@ -36,8 +36,10 @@ EXPORTS = (
     "dwfgpu_matpcdagmatpc", "dwfgpu_matpc", "dwfgpu_dslash", "dwfgpu_invcg",
     "dwfgpu_matinv", "dwfgpu_vec_create", "dwfgpu_vec_destroy", "dwfgpu_vec_upload",
     "dwfgpu_vec_download", "dwfgpu_matpcdagmatpc_dev", "dwfgpu_invcg_dev",
-    "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost",
+    "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
 )
+# every entry point of include/dwf_synth.h
+SYNTH_EXPORTS = ("dwfsynth_gauge", "dwfsynth_gaussian")
 
 
 class DwfGpuError(RuntimeError):
@@ -99,6 +101,11 @@ def lib():
                                                 C.POINTER(C.c_int)]
         L.dwfgpu_matpcdagmatpc_cost.argtypes = [H, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double)]
+        L.dwfgpu_time_copy.argtypes = [H, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.dwfsynth_gauge.argtypes = [_dp, C.c_int * 4, C.c_double, C.c_ulonglong]
+        L.dwfsynth_gauge.restype = None
+        L.dwfsynth_gaussian.argtypes = [_dp, C.c_size_t, C.c_ulonglong]
+        L.dwfsynth_gaussian.restype = None
         _lib = L
     return _lib
 
@@ -225,6 +232,11 @@ class _DiracOpGpu:
         _chk(lib().dwfgpu_time_matpcdagmatpc(self.h, mass, out.h, inp.h, reps, C.byref(ms),
                                              msk, byk, C.byref(nk)), "time_MatPcDagMatPc")
         return ms.value, msk[:nk.value], byk[:nk.value]
+
+    def time_copy(self, nbytes, reps=10):
+        g = C.c_double()
+        _chk(lib().dwfgpu_time_copy(self.h, int(nbytes), reps, C.byref(g)), "time_copy")
+        return g.value
 
     def cost(self):
         b, f = C.c_double(), C.c_double()

@@ -1,10 +1,9 @@
 """Synthetic lattice inputs and CPS storage orders (host side, numpy).
 
 Gauge links follow the recipe ``U = reunitarize(exp(i*eps*H))`` with ``H`` a
-seeded Gaussian traceless Hermitian 3x3 matrix per link, and reunitarize the
-row-wise Gram-Schmidt of CPS's ``Matrix::Unitarize``
-(cps_pp/src/util/vector/comsrc/unitarize.C:28-56): normalise row 0,
-orthogonalise and normalise row 1, row 2 = conj(row0 x row1).
+seeded Gaussian traceless Hermitian 3x3 matrix per link (generated in C++, see
+include/dwf_synth.h), reunitarize being CPS's ``Matrix::Unitarize``
+(cps_pp/src/util/vector/comsrc/unitarize.C:28-56).
 
 Storage orders (CPS's own):
 
@@ -22,63 +21,19 @@ Storage orders (CPS's own):
 """
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
-
-_GELL_MANN = np.zeros((8, 3, 3), dtype=np.complex128)
-_GELL_MANN[0, 0, 1] = _GELL_MANN[0, 1, 0] = 1
-_GELL_MANN[1, 0, 1], _GELL_MANN[1, 1, 0] = -1j, 1j
-_GELL_MANN[2, 0, 0], _GELL_MANN[2, 1, 1] = 1, -1
-_GELL_MANN[3, 0, 2] = _GELL_MANN[3, 2, 0] = 1
-_GELL_MANN[4, 0, 2], _GELL_MANN[4, 2, 0] = -1j, 1j
-_GELL_MANN[5, 1, 2] = _GELL_MANN[5, 2, 1] = 1
-_GELL_MANN[6, 1, 2], _GELL_MANN[6, 2, 1] = -1j, 1j
-_GELL_MANN[7] = np.diag([1, 1, -2]) / np.sqrt(3)
-
-
-def _rng(seed: int) -> np.random.Generator:
-    return np.random.Generator(np.random.Philox(seed))
-
-
-def expm_su3_algebra(x: np.ndarray) -> np.ndarray:
-    """exp(x) for a stack of 3x3 anti-Hermitian matrices (scaling+squaring)."""
-    norm = np.max(np.abs(x).sum(axis=-1), axis=-1)
-    nsq = int(max(0, np.ceil(np.log2(max(float(norm.max(initial=0.0)), 1e-300)))) + 4)
-    y = x / (2.0 ** nsq)
-    out = np.broadcast_to(np.eye(3, dtype=np.complex128), x.shape).copy()
-    term = out.copy()
-    for k in range(1, 18):
-        term = term @ y / k
-        out += term
-    for _ in range(nsq):
-        out = out @ out
-    return out
-
-
-def reunitarize(u: np.ndarray) -> np.ndarray:
-    """Matrix::Unitarize on a stack of 3x3 matrices (unitarize.C:28-56)."""
-    u = u.copy()
-    r0 = u[..., 0, :]
-    r0 /= np.sqrt(np.sum(np.abs(r0) ** 2, axis=-1, keepdims=True))
-    r1 = u[..., 1, :]
-    r1 -= r0 * np.sum(np.conj(r0) * r1, axis=-1, keepdims=True)
-    r1 /= np.sqrt(np.sum(np.abs(r1) ** 2, axis=-1, keepdims=True))
-    u[..., 2, :] = np.conj(np.cross(r0, r1))
-    return u
 
 
 def random_gauge(dims, eps: float, seed: int) -> np.ndarray:
-    """CANONICAL gauge field, complex array of shape (lt,lz,ly,lx,4,3,3)."""
+    """CANONICAL gauge field, complex array of shape (lt,lz,ly,lx,4,3,3),
+    from the library's counter-based generator (include/dwf_synth.h)."""
+    from .dwf import lib
     lx, ly, lz, lt = dims
-    n = lx * ly * lz * lt * 4
-    rng = _rng(seed)
-    out = np.empty((n, 3, 3), dtype=np.complex128)
-    chunk = 1 << 20
-    for lo in range(0, n, chunk):
-        hi = min(n, lo + chunk)
-        w = rng.standard_normal((hi - lo, 8))
-        h = np.einsum("na,aij->nij", w, _GELL_MANN) * 0.5
-        out[lo:hi] = reunitarize(expm_su3_algebra(1j * eps * h))
-    return out.reshape(lt, lz, ly, lx, 4, 3, 3)
+    out = np.empty(lx * ly * lz * lt * 4 * 18)
+    lib().dwfsynth_gauge(out, (ctypes.c_int * 4)(*dims), float(eps), int(seed))
+    return out.view(np.complex128).reshape(lt, lz, ly, lx, 4, 3, 3)
 
 
 def unit_gauge(dims) -> np.ndarray:
@@ -95,10 +50,11 @@ def gauge_reals(u: np.ndarray) -> np.ndarray:
 
 def gaussian_spinor(shape, seed: int) -> np.ndarray:
     """Seeded unit Gaussian complex spinor of the given leading shape (...,4,3)."""
-    rng = _rng(seed)
-    re = rng.standard_normal(tuple(shape) + (4, 3))
-    im = rng.standard_normal(tuple(shape) + (4, 3))
-    return re + 1j * im
+    from .dwf import lib
+    n = int(np.prod(shape)) * 12
+    out = np.empty(2 * n)
+    lib().dwfsynth_gaussian(out, out.size, int(seed))
+    return out.view(np.complex128).reshape(tuple(shape) + (4, 3))
 
 
 # ---------------------------------------------------------------------------

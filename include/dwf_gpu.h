@@ -115,6 +115,10 @@ int dwfgpu_time_matpcdagmatpc(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_
 /* Algorithmic bytes and flops of one MatPcDagMatPc application with the
  * handle's storage precision (see include/roofline.h).                      */
 int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, double *bytes, double *flops);
+/* Reference copy kernel on the handle's stream: copies `bytes` (rounded
+ * down to 16) between two fresh device buffers `reps` times; *gbs receives
+ * (read + write bytes) / mean kernel time.                                   */
+int dwfgpu_time_copy(dwfgpu_t h, size_t bytes, int reps, double *gbs);
 
 #ifdef __cplusplus
 }

@@ -11,21 +11,25 @@ from cps_veloc_amd import dwf
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared():
-    text = open(os.path.join(ROOT, "include", "dwf_gpu.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(dwfgpu_\w+)\s*\(", text)))
+def declared(header=None):
+    names = set()
+    for h in ([header] if header else ["dwf_gpu.h", "dwf_synth.h"]):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(dwf(?:gpu|synth)_\w+)\s*\(", text))
+    return sorted(names)
 
 
 def test_header_matches_binding_list():
-    assert declared() == sorted(dwf.EXPORTS)
+    assert declared("dwf_gpu.h") == sorted(dwf.EXPORTS)
+    assert declared("dwf_synth.h") == sorted(dwf.SYNTH_EXPORTS)
 
 
 @pytest.mark.skipif(not os.path.exists(dwf.LIB_PATH), reason="library not built")
 def test_library_exports_every_declared_symbol():
     out = subprocess.run(["nm", "-D", "--defined-only", dwf.LIB_PATH], check=True,
                          capture_output=True, text=True).stdout
-    exported = set(re.findall(r"\bT\s+(dwfgpu_\w+)", out))
+    exported = set(re.findall(r"\bT\s+(dwf(?:gpu|synth)_\w+)", out))
     missing = [s for s in declared() if s not in exported]
     assert not missing, missing
     lib = dwf.lib()
