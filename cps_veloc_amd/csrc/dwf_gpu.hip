@@ -6,6 +6,7 @@
 #include <complex>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -47,7 +48,6 @@ void set_err(const char* fmt, ...) {
     if (r_ != DWFGPU_OK) return r_;   \
   } while (0)
 
-constexpr int NS = 16;  // checkerboard sites per work-group (x) ; y = all slices
 
 }  // namespace
 
@@ -70,6 +70,8 @@ struct dwfgpu_s {
   double* dscal = nullptr;
   double* hscal = nullptr;  // pinned
   int grid_hop = 0;
+  int ns = 16;   // sites per work-group of the hopping kernels (x); y = all slices
+  int wpe = 2;   // minimum waves per SIMD the hopping kernels are compiled for
 };
 
 struct dwfgpu_vec_s {
@@ -158,15 +160,24 @@ MobArgs mob_args(const dwfgpu_s* h, double mass, int dag) {
   return a;
 }
 
-size_t lds_bytes(const dwfgpu_s* h) {
-  return (size_t)NS * (h->ls * 12 + 1) * 2 * h->elem;
+// LDS of the s-column buffer (M5^-1 and (b + c D5)^dag epilogues)
+size_t col_lds_bytes(const dwfgpu_s* h) { return (size_t)h->ns * (h->ls * 12 + 1) * 2 * h->elem; }
+// LDS of the staged links: nsets parities x 8 directions x ns sites x 9
+size_t link_lds_bytes(const dwfgpu_s* h, int nsets) {
+  return (size_t)nsets * 8 * h->ns * 9 * 2 * h->elem;
+}
+// largest LDS request of any kernel of this handle
+size_t max_lds_bytes(const dwfgpu_s* h) {
+  const size_t dwf = link_lds_bytes(h, 2);
+  const size_t mob = link_lds_bytes(h, 1) + col_lds_bytes(h);
+  return h->action == DWFGPU_DWF ? dwf : mob;
 }
 
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
 int fetch_sum(dwfgpu_s* h, int nparts, double* out) {
-  hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(256), 0, h->st, h->partial, nparts,
+  hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(1024), 0, h->st, h->partial, nparts,
                      h->dscal);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(h->hscal, h->dscal, sizeof(double), hipMemcpyDeviceToHost, h->st));
@@ -188,15 +199,22 @@ struct Ops {
   // DWF hop kernel: epi 0 -> out = D in ; epi 1 -> out = x0 + alpha D in
   static int dwf(dwfgpu_s* h, double mass, void* out, const void* in, int cb_in, int dag,
                  int epi, const void* x0, double alpha, bool norm) {
-    const dim3 grid(h->grid_hop), block(NS, h->ls);
+    const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const double two_a5 = 2.0 * h->a5inv, nm = -2.0 * mass * h->a5inv;
     C* o = (C*)out;
     const C* i = (const C*)in;
     const C* u = (const C*)h->U;
     const C* x = (const C*)x0;
+    const size_t lds = link_lds_bytes(h, 2);
 #define DWF_LAUNCH(DAG, EPI, NORM)                                                              \
-  hipLaunchKernelGGL((dwfk::k_dwf_dslash<R, DAG, EPI, NORM>), grid, block, 0, h->st, o, i, u, x, \
-                     h->partial, h->g, cb_in, two_a5, nm, alpha)
+  do {                                                                                          \
+    if (h->wpe >= 4)                                                                            \
+      hipLaunchKernelGGL((dwfk::k_dwf_dslash<R, DAG, EPI, NORM, 4>), grid, block, lds, h->st, o,  \
+                         i, u, x, h->partial, h->g, cb_in, two_a5, nm, alpha);                  \
+    else                                                                                        \
+      hipLaunchKernelGGL((dwfk::k_dwf_dslash<R, DAG, EPI, NORM, 2>), grid, block, lds, h->st, o,  \
+                         i, u, x, h->partial, h->g, cb_in, two_a5, nm, alpha);                  \
+  } while (0)
     if (epi == 0) {
       if (dag) DWF_LAUNCH(1, 0, false); else DWF_LAUNCH(0, 0, false);
     } else if (norm) {
@@ -209,41 +227,70 @@ struct Ops {
     return DWFGPU_OK;
   }
 
+  // Mobius hop kernel; fl: dwfk::FL_* flags
   static int mob(dwfgpu_s* h, double mass, void* out, const void* in, const void* psi, int p_out,
-                 int dag, int epi, bool norm) {
-    const dim3 grid(h->grid_hop), block(NS, h->ls);
+                 int dag, int epi, int fl) {
+    const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const MobArgs a = mob_args(h, mass, dag);
     C* o = (C*)out;
     const C* i = (const C*)in;
     const C* p = (const C*)psi;
     const C* u = (const C*)h->U;
-    const size_t lds = (dag || epi == dwfk::EPI_M5INV) ? lds_bytes(h) : 0;
-#define MOB_LAUNCH(DAG, EPI, NORM)                                                             \
-  hipLaunchKernelGGL((dwfk::k_mobius<R, DAG, EPI, NORM>), grid, block, lds, h->st, o, i, p, u, \
-                     h->partial, h->g, p_out, a)
-    if (dag == 0) {
-      if (epi == dwfk::EPI_M5INV) MOB_LAUNCH(0, dwfk::EPI_M5INV, false);
-      else if (epi == dwfk::EPI_SCHUR && norm) MOB_LAUNCH(0, dwfk::EPI_SCHUR, true);
-      else if (epi == dwfk::EPI_SCHUR) MOB_LAUNCH(0, dwfk::EPI_SCHUR, false);
-      else MOB_LAUNCH(0, dwfk::EPI_PLAIN, false);
+    const bool col = dag || epi == dwfk::EPI_M5INV;
+    const size_t lds = link_lds_bytes(h, 1) + (col ? col_lds_bytes(h) : 0);
+#define MOB_LAUNCH(DAG, EPI, FL)                                                               \
+  do {                                                                                         \
+    if (h->wpe >= 4)                                                                           \
+      hipLaunchKernelGGL((dwfk::k_mobius<R, DAG, EPI, FL, 4>), grid, block, lds, h->st, o, i, p, \
+                         u, h->partial, h->g, p_out, a);                                       \
+    else                                                                                       \
+      hipLaunchKernelGGL((dwfk::k_mobius<R, DAG, EPI, FL, 2>), grid, block, lds, h->st, o, i, p, \
+                         u, h->partial, h->g, p_out, a);                                       \
+  } while (0)
+    using dwfk::FL_NORM;
+    using dwfk::FL_POST_T;
+    using dwfk::FL_PRE_T;
+    if (dag == 0 && epi == dwfk::EPI_M5INV) {
+      if (fl & FL_POST_T) MOB_LAUNCH(0, dwfk::EPI_M5INV, FL_POST_T);
+      else MOB_LAUNCH(0, dwfk::EPI_M5INV, 0);
+    } else if (dag == 0 && epi == dwfk::EPI_SCHUR) {
+      if ((fl & FL_PRE_T) && (fl & FL_NORM)) MOB_LAUNCH(0, dwfk::EPI_SCHUR, FL_PRE_T | FL_NORM);
+      else if (fl & FL_PRE_T) MOB_LAUNCH(0, dwfk::EPI_SCHUR, FL_PRE_T);
+      else if (fl & FL_NORM) MOB_LAUNCH(0, dwfk::EPI_SCHUR, FL_NORM);
+      else MOB_LAUNCH(0, dwfk::EPI_SCHUR, 0);
+    } else if (dag == 0) {
+      MOB_LAUNCH(0, dwfk::EPI_PLAIN, 0);
+    } else if (epi == dwfk::EPI_M5INV) {
+      MOB_LAUNCH(1, dwfk::EPI_M5INV, 0);
+    } else if (epi == dwfk::EPI_SCHUR) {
+      MOB_LAUNCH(1, dwfk::EPI_SCHUR, 0);
     } else {
-      if (epi == dwfk::EPI_M5INV) MOB_LAUNCH(1, dwfk::EPI_M5INV, false);
-      else if (epi == dwfk::EPI_SCHUR) MOB_LAUNCH(1, dwfk::EPI_SCHUR, false);
-      else MOB_LAUNCH(1, dwfk::EPI_PLAIN, false);
+      MOB_LAUNCH(1, dwfk::EPI_PLAIN, 0);
     }
 #undef MOB_LAUNCH
     HIPCHK(hipGetLastError());
     return DWFGPU_OK;
   }
 
+  // first / second half of MatPc (dag 0) or MatPcDag (dag 1): the first writes
+  // h->te (for dag 0 already multiplied by (b + c D5), consumed by the second)
+  static int mob_first(dwfgpu_s* h, double mass, void* te, const void* in, int dag) {
+    return mob(h, mass, te, in, nullptr, 0, dag, dwfk::EPI_M5INV, dag ? 0 : dwfk::FL_POST_T);
+  }
+  static int mob_second(dwfgpu_s* h, double mass, void* out, const void* te, const void* in,
+                        int dag, bool norm) {
+    const int fl = (dag ? 0 : dwfk::FL_PRE_T) | (norm && !dag ? dwfk::FL_NORM : 0);
+    return mob(h, mass, out, te, in, 1, dag, dwfk::EPI_SCHUR, fl);
+  }
+
   static int m5inv(dwfgpu_s* h, double mass, void* out, const void* in, int dag) {
-    const dim3 grid(h->grid_hop), block(NS, h->ls);
+    const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const M5Side k = m5_side(h, mass, dag);
     if (dag)
-      hipLaunchKernelGGL((dwfk::k_m5inv<R, 1>), grid, block, lds_bytes(h), h->st, (C*)out,
+      hipLaunchKernelGGL((dwfk::k_m5inv<R, 1>), grid, block, col_lds_bytes(h), h->st, (C*)out,
                          (const C*)in, h->g, k);
     else
-      hipLaunchKernelGGL((dwfk::k_m5inv<R, 0>), grid, block, lds_bytes(h), h->st, (C*)out,
+      hipLaunchKernelGGL((dwfk::k_m5inv<R, 0>), grid, block, col_lds_bytes(h), h->st, (C*)out,
                          (const C*)in, h->g, k);
     HIPCHK(hipGetLastError());
     return DWFGPU_OK;
@@ -256,8 +303,8 @@ struct Ops {
       TRY(dwf(h, mass, h->te, in, 1, dag, 0, nullptr, 0.0, false));
       TRY(dwf(h, mass, out, h->te, 0, dag, 1, in, mk2, norm));
     } else {
-      TRY(mob(h, mass, h->te, in, nullptr, 0, dag, dwfk::EPI_M5INV, false));
-      TRY(mob(h, mass, out, h->te, in, 1, dag, dwfk::EPI_SCHUR, norm && !dag));
+      TRY(mob_first(h, mass, h->te, in, dag));
+      TRY(mob_second(h, mass, out, h->te, in, dag, norm));
     }
     return DWFGPU_OK;
   }
@@ -271,7 +318,7 @@ struct Ops {
 
   static int dslash(dwfgpu_s* h, double mass, void* out, const void* in, int cb, int dag) {
     if (h->action == DWFGPU_DWF) return dwf(h, mass, out, in, cb, dag, 0, nullptr, 0.0, false);
-    return mob(h, mass, out, in, nullptr, 1 - cb, dag, dwfk::EPI_PLAIN, false);
+    return mob(h, mass, out, in, nullptr, 1 - cb, dag, dwfk::EPI_PLAIN, 0);
   }
 
   static int norm(dwfgpu_s* h, const void* x, double* out) {
@@ -376,12 +423,12 @@ struct Ops {
       const double kb = mob_kappa_b(h);
       void* temp2 = h->work[5];
       TRY(m5inv(h, mass, temp, in_first, 0));
-      TRY(mob(h, mass, temp2, temp, nullptr, 1, 0, dwfk::EPI_PLAIN, false));
+      TRY(mob(h, mass, temp2, temp, nullptr, 1, 0, dwfk::EPI_PLAIN, 0));
       TRY(axpy_to(h, temp, temp2, in_second, kb));
       TRY(matpc(h, mass, src, temp, 1, false));
       st = invcg(h, arg, out_second, src, true_res, iters);
       if (st != DWFGPU_OK && st != DWFGPU_MAXITER) return st;
-      TRY(mob(h, mass, temp, out_second, nullptr, 0, 0, dwfk::EPI_PLAIN, false));
+      TRY(mob(h, mass, temp, out_second, nullptr, 0, 0, dwfk::EPI_PLAIN, 0));
       TRY(scale(h, temp, kb));
       TRY(m5inv(h, mass, out_first, temp, 0));
       TRY(m5inv(h, mass, temp, in_first, 0));
@@ -396,14 +443,14 @@ struct Ops {
     HIPCHK(hipMemcpyAsync(h->cps_stage, host, h->n * sizeof(double), hipMemcpyHostToDevice,
                           h->st));
     hipLaunchKernelGGL((dwfk::k_cps_to_soa<R>), dim3(2048), dim3(256), 0, h->st, (C*)dst,
-                       (const double*)h->cps_stage, h->g.vh, h->g.ls);
+                       (const double*)h->cps_stage, h->g);
     HIPCHK(hipGetLastError());
     return DWFGPU_OK;
   }
 
   static int to_host(dwfgpu_s* h, double* host, const void* src) {
     hipLaunchKernelGGL((dwfk::k_soa_to_cps<R>), dim3(2048), dim3(256), 0, h->st,
-                       (double*)h->cps_stage, (const C*)src, h->g.vh, h->g.ls);
+                       (double*)h->cps_stage, (const C*)src, h->g);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(host, h->cps_stage, h->n * sizeof(double), hipMemcpyDeviceToHost,
                           h->st));
@@ -438,7 +485,7 @@ int upload_gauge(dwfgpu_s* h, const double* uc, const int bc[4]) {
         for (int x = 0; x < lx; ++x) {
           const size_t site = x + (size_t)lx * (y + (size_t)ly * (z + (size_t)lz * t));
           const int par = (x + y + z + t) & 1;
-          const size_t cbs = x / 2 + (size_t)(lx / 2) * (y + (size_t)ly * (z + (size_t)lz * t));
+          const size_t cbs = dwfk::site_index(h->g, x / 2, y, z, t);
           const int pos[4] = {x, y, z, t};
           for (int mu = 0; mu < 4; ++mu) {
             const double sg = (bc[mu] && pos[mu] == h->dims[mu] - 1) ? -1.0 : 1.0;
@@ -478,19 +525,19 @@ int time_mdagm(dwfgpu_s* h, double mass, void* out, const void* in, int reps, do
     const size_t b = (size_t)r * nk;
     rec(b + 0);
     if (!st) st = dwf ? O::dwf(h, mass, h->te, in, 1, 0, 0, nullptr, 0, false)
-                      : O::mob(h, mass, h->te, in, nullptr, 0, 0, dwfk::EPI_M5INV, false);
+                      : O::mob_first(h, mass, h->te, in, 0);
     rec(b + 1);
     if (!st) st = dwf ? O::dwf(h, mass, h->t1, h->te, 0, 0, 1, in, mk2, true)
-                      : O::mob(h, mass, h->t1, h->te, in, 1, 0, dwfk::EPI_SCHUR, true);
+                      : O::mob_second(h, mass, h->t1, h->te, in, 0, true);
     rec(b + 2);
     if (!st) st = dwf ? O::dwf(h, mass, h->te, h->t1, 1, 1, 0, nullptr, 0, false)
-                      : O::mob(h, mass, h->te, h->t1, nullptr, 0, 1, dwfk::EPI_M5INV, false);
+                      : O::mob_first(h, mass, h->te, h->t1, 1);
     rec(b + 3);
     if (!st) st = dwf ? O::dwf(h, mass, out, h->te, 0, 1, 1, h->t1, mk2, false)
-                      : O::mob(h, mass, out, h->te, h->t1, 1, 1, dwfk::EPI_SCHUR, false);
+                      : O::mob_second(h, mass, out, h->te, h->t1, 1, false);
     rec(b + 4);
     if (!st) {
-      hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(256), 0, h->st, h->partial,
+      hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(1024), 0, h->st, h->partial,
                          h->grid_hop, h->dscal);
       if (hipGetLastError() != hipSuccess) st = DWFGPU_HIP_ERROR;
     }
@@ -580,9 +627,40 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
     set_err("dwfgpu_create: local volume too large");
     return DWFGPU_BAD_ARG;
   }
-  h->g = Geom{dims[0], dims[1], dims[2], dims[3], dims[0] / 2, (int)(v4 / 2), ls};
+  // t tiles of 4 slices when possible; 8 XCDs on gfx950 (a different count
+  // only changes the work-group order, not the result)
+  const int bt = dims[3] % 4 == 0 ? 4 : 2;
+  const int bz = dims[2] % 4 == 0 ? 4 : 2;
+  h->g = Geom{dims[0], dims[1], dims[2], dims[3], dims[0] / 2, (int)(v4 / 2), ls, bt, bz, 8};
   h->n = (size_t)24 * (v4 / 2) * ls;
-  h->grid_hop = (h->g.vh + NS - 1) / NS;
+  int dev = 0, max_lds = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+          hipSuccess) {
+    set_err("dwfgpu_create: no HIP device");
+    delete h;
+    return DWFGPU_HIP_ERROR;
+  }
+  // launch shape of the hopping kernels; DWFGPU_NS / DWFGPU_WPE override
+  // the defaults (sites per work-group, minimum waves per SIMD).  Without an
+  // override, fewer sites per work-group are taken when the LDS stage of a
+  // long fifth dimension would not fit.
+  const char* env_ns = getenv("DWFGPU_NS");
+  if (env_ns) h->ns = atoi(env_ns);
+  if (const char* e = getenv("DWFGPU_WPE")) h->wpe = atoi(e);
+  while (!env_ns && h->ns > 4 && max_lds_bytes(h) > (size_t)max_lds) h->ns /= 2;
+  if (h->ns < 1 || h->ns * ls > 512 || (h->wpe != 2 && h->wpe != 4)) {
+    set_err("dwfgpu_create: bad launch shape ns=%d (ns*Ls <= 512) wpe=%d (2 or 4)", h->ns, h->wpe);
+    delete h;
+    return DWFGPU_BAD_ARG;
+  }
+  if (max_lds_bytes(h) > (size_t)max_lds) {
+    set_err("dwfgpu_create: Ls=%d needs %zu bytes of LDS per work-group (limit %d)", ls,
+            max_lds_bytes(h), max_lds);
+    delete h;
+    return DWFGPU_BAD_ARG;
+  }
+  h->grid_hop = (h->g.vh + h->ns - 1) / h->ns;
   h->npartial = h->grid_hop > dwfk::BLAS_BLOCKS ? h->grid_hop : dwfk::BLAS_BLOCKS;
   int r = DWFGPU_OK;
   auto fail = [&](int code) {
@@ -590,22 +668,6 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
     delete h;
     return code;
   };
-  {
-    int dev = 0, max_lds = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
-            hipSuccess) {
-      set_err("dwfgpu_create: no HIP device");
-      delete h;
-      return DWFGPU_HIP_ERROR;
-    }
-    if (lds_bytes(h) > (size_t)max_lds) {
-      set_err("dwfgpu_create: Ls=%d needs %zu bytes of LDS per work-group (limit %d)", ls,
-              lds_bytes(h), max_lds);
-      delete h;
-      return DWFGPU_BAD_ARG;
-    }
-  }
   if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
     set_err("dwfgpu_create: hipStreamCreate failed");
     delete h;

@@ -2,13 +2,18 @@
 // domain-wall / Mobius operator.
 //
 // Device layout (structure of arrays, one complex number per 8/16-byte word):
-//   half vector  v[((s*12 + c)*vh + i)]     c = spin*3 + colour, i = cb site
+//   half vector  v[((s*12 + c)*vh + i)]     c = spin*3 + colour, i = site index
 //   gauge        U[((mu*2 + parity)*9 + row*3 + col)*vh + i]   (canonical U(row,col),
 //                boundary signs already folded in)
+// Site index of a checkerboard site (z- and t-tiled, fastest first):
+//   i = x/2 + lx/2*(t%bt + bt*(z%bz + bz*(y + ly*(z/bz + lz/bz*(t/bt)))))
+// so y-, z- and t-hops inside a bz x bt tile stay within a few hundred sites
+// and the neighbours of a work-group stay within one XCD's L2.
 // A thread owns one output site (i, s); a work-group holds NS consecutive
-// checkerboard sites times all Ls slices, so the 8 links of a site are fetched
-// once per group and served to all slices from L1/L2, and a whole s-column of
-// a site sits in one group (needed by the Mobius M5^-1 recurrence, done in LDS).
+// sites times all Ls slices, so the 8 links of a site are fetched once per
+// group and served to all slices from L1/L2, and a whole s-column of a site
+// sits in one group (needed by the Mobius M5^-1 recurrence, done in LDS).
+// Work-groups are renumbered so each XCD sweeps one contiguous range.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -49,35 +54,71 @@ __device__ __forceinline__ cx<R> sgr(cx<R> a) {
 
 struct Geom {
   int lx, ly, lz, lt, lxh, vh, ls;
+  int bt, bz;  // time / z slices per tile of the site index (divide lt / lz)
+  int nxcd;    // XCDs the work-groups are spread over
 };
+
+// site index of (x/2, y, z, t): order (fastest first) x/2, t%bt, z%bz, y,
+// z/bz, t/bt
+__host__ __device__ __forceinline__ int site_index(const Geom& g, int xh, int y, int z, int t) {
+  const int tb = t / g.bt, ti = t - tb * g.bt;
+  const int zb = z / g.bz, zi = z - zb * g.bz;
+  return xh + g.lxh * (ti + g.bt * (zi + g.bz * (y + g.ly * (zb + (g.lz / g.bz) * tb))));
+}
+
+// site index of a CPS checkerboard index (x/2) + (lx/2)(y + ly(z + lz t))
+__host__ __device__ __forceinline__ int site_of_cps(const Geom& g, int c) {
+  const int xh = c % g.lxh;
+  int r = c / g.lxh;
+  const int y = r % g.ly;
+  r /= g.ly;
+  const int z = r % g.lz;
+  const int t = r / g.lz;
+  return site_index(g, xh, y, z, t);
+}
 
 struct Nbr {
   int fwd[4], bwd[4];
 };
 
-__device__ __forceinline__ Nbr neighbours(const Geom& g, int i, int parity) {
+__host__ __device__ __forceinline__ Nbr neighbours(const Geom& g, int i, int parity) {
   const int xh = i % g.lxh;
   int r = i / g.lxh;
+  const int ti = r % g.bt;
+  r /= g.bt;
+  const int zi = r % g.bz;
+  r /= g.bz;
   const int y = r % g.ly;
   r /= g.ly;
-  const int z = r % g.lz;
-  const int t = r / g.lz;
+  const int nzb = g.lz / g.bz;
+  const int z = (r % nzb) * g.bz + zi;
+  const int t = (r / nzb) * g.bt + ti;
   const int x = 2 * xh + ((parity + y + z + t) & 1);
   const int xp = (x + 1 == g.lx) ? 0 : x + 1, xm = (x == 0) ? g.lx - 1 : x - 1;
   const int yp = (y + 1 == g.ly) ? 0 : y + 1, ym = (y == 0) ? g.ly - 1 : y - 1;
   const int zp = (z + 1 == g.lz) ? 0 : z + 1, zm = (z == 0) ? g.lz - 1 : z - 1;
   const int tp = (t + 1 == g.lt) ? 0 : t + 1, tm = (t == 0) ? g.lt - 1 : t - 1;
-  const int yzt = g.lxh * (y + g.ly * (z + g.lz * t));
+  const int row = i - xh;  // index of (0, y, z, t)
   Nbr n;
-  n.fwd[0] = (xp >> 1) + yzt;
-  n.bwd[0] = (xm >> 1) + yzt;
-  n.fwd[1] = xh + g.lxh * (yp + g.ly * (z + g.lz * t));
-  n.bwd[1] = xh + g.lxh * (ym + g.ly * (z + g.lz * t));
-  n.fwd[2] = xh + g.lxh * (y + g.ly * (zp + g.lz * t));
-  n.bwd[2] = xh + g.lxh * (y + g.ly * (zm + g.lz * t));
-  n.fwd[3] = xh + g.lxh * (y + g.ly * (z + g.lz * tp));
-  n.bwd[3] = xh + g.lxh * (y + g.ly * (z + g.lz * tm));
+  n.fwd[0] = (xp >> 1) + row;
+  n.bwd[0] = (xm >> 1) + row;
+  n.fwd[1] = site_index(g, xh, yp, z, t);
+  n.bwd[1] = site_index(g, xh, ym, z, t);
+  n.fwd[2] = site_index(g, xh, y, zp, t);
+  n.bwd[2] = site_index(g, xh, y, zm, t);
+  n.fwd[3] = site_index(g, xh, y, z, tp);
+  n.bwd[3] = site_index(g, xh, y, z, tm);
   return n;
+}
+
+// Work-groups are dispatched round-robin over the XCDs (hardware id b runs on
+// XCD b % nxcd).  Renumber them so that XCD k receives one contiguous range of
+// logical ids; neighbouring sites then share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int nxcd) {
+  const int b = blockIdx.x, nb = gridDim.x;
+  const int q = nb / nxcd, r = nb - q * nxcd;
+  const int k = b % nxcd, j = b / nxcd;
+  return k * q + (k < r ? k : r) + j;
 }
 
 // (1 + SG gamma_mu) psi -> two-spinor h (spins 0,1), gamma matrices of the
@@ -153,18 +194,49 @@ struct MobTLoader {
   }
 };
 
+// Links of the work-group's sites, staged once in LDS and shared by all Ls
+// slices: lu[((set*8 + d)*ns + site)*9 + e], d < 4 the forward link U_d(x),
+// d >= 4 the backward link U_{d-4}(x - d) of the output site x; `set`
+// distinguishes the two output parities of a DWF work-group.
+template <typename R>
+__device__ __forceinline__ void stage_links(cx<R>* __restrict__ lu, const cx<R>* __restrict__ U,
+                                            const Geom& g, int blk, int ns, int nsets,
+                                            int p_first) {
+  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+  const int nthr = blockDim.x * blockDim.y;
+  for (int w = tid; w < nsets * 8 * ns; w += nthr) {
+    const int il = w % ns, d = (w / ns) % 8, set = w / (8 * ns);
+    const int i = blk * ns + il;
+    if (i >= g.vh) continue;
+    const int p = (p_first + set) & 1;
+    int site, par;
+    if (d < 4) {
+      site = i;
+      par = p;
+    } else {
+      site = neighbours(g, i, p).bwd[d - 4];
+      par = 1 - p;
+    }
+    const int mu = d & 3;
+    const cx<R>* src = U + (size_t)((mu * 2 + par) * 9) * g.vh + site;
+    cx<R>* dst = lu + (size_t)w * 9;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) dst[e] = src[(size_t)e * g.vh];
+  }
+}
+
+// lf / lb: this site's forward / backward link of direction MU (9 elements)
 template <int MU, int DAG, typename R, class LD>
-__device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* __restrict__ U,
-                                        const Geom& g, int i, int p_out, const Nbr& nb) {
+__device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* lf,
+                                        const cx<R>* lb, const Nbr& nb) {
   constexpr int SF = DAG ? 1 : -1;  // forward: 1 - s gamma, s = +1 (dag 0) / -1 (dag 1)
   cx<R> psi[12], h[6], r[6];
   {
     ld(nb.fwd[MU], psi);
     project<MU, SF>(psi, h);
-    const cx<R>* u = U + (size_t)((MU * 2 + p_out) * 9) * g.vh + i;
     cx<R> m[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) m[e] = u[e * g.vh];
+    for (int e = 0; e < 9; ++e) m[e] = lf[e];
 #pragma unroll
     for (int k = 0; k < 2; ++k)
 #pragma unroll
@@ -174,13 +246,11 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* _
     reconstruct<MU, SF>(acc, r);
   }
   {
-    const int n = nb.bwd[MU];
-    ld(n, psi);
+    ld(nb.bwd[MU], psi);
     project<MU, -SF>(psi, h);
-    const cx<R>* u = U + (size_t)((MU * 2 + (1 - p_out)) * 9) * g.vh + n;
     cx<R> m[9];
 #pragma unroll
-    for (int e = 0; e < 9; ++e) m[e] = u[e * g.vh];
+    for (int e = 0; e < 9; ++e) m[e] = lb[e];
 #pragma unroll
     for (int k = 0; k < 2; ++k)
 #pragma unroll
@@ -191,15 +261,16 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* _
   }
 }
 
-// acc = D_hop^(dag) psi at output site i of parity p_out (wilson_dslash.C:173-517)
+// acc = D_hop^(dag) psi at output site i of parity p_out (wilson_dslash.C:173-517);
+// lu points at this site's 8 staged links, consecutive directions `ds` apart.
 template <int DAG, typename R, class LD>
-__device__ __forceinline__ void hop4(cx<R>* acc, const LD& ld, const cx<R>* __restrict__ U,
+__device__ __forceinline__ void hop4(cx<R>* acc, const LD& ld, const cx<R>* lu, int ds,
                                      const Geom& g, int i, int p_out) {
   const Nbr nb = neighbours(g, i, p_out);
-  hop_dir<0, DAG>(acc, ld, U, g, i, p_out, nb);
-  hop_dir<1, DAG>(acc, ld, U, g, i, p_out, nb);
-  hop_dir<2, DAG>(acc, ld, U, g, i, p_out, nb);
-  hop_dir<3, DAG>(acc, ld, U, g, i, p_out, nb);
+  hop_dir<0, DAG>(acc, ld, lu + 0 * ds, lu + 4 * ds, nb);
+  hop_dir<1, DAG>(acc, ld, lu + 1 * ds, lu + 5 * ds, nb);
+  hop_dir<2, DAG>(acc, ld, lu + 2 * ds, lu + 6 * ds, nb);
+  hop_dir<3, DAG>(acc, ld, lu + 3 * ds, lu + 7 * ds, nb);
 }
 
 // sum over the work-group; result valid in thread 0.  Fixed order, so the
@@ -222,14 +293,20 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // (dwf_dslash_4.C:107-137, dwf_dslash_5_plus_omp.C:115-270).
 // EPI 0: out = D in.   EPI 1: out = x0 + alpha D in.  NORM: |out|^2 partials.
 // ---------------------------------------------------------------------------
-template <typename R, int DAG, int EPI, bool NORM>
-__global__ void __launch_bounds__(512)
+template <typename R, int DAG, int EPI, bool NORM, int WPE>
+__global__ void __launch_bounds__(512, WPE)
 k_dwf_dslash(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __restrict__ U,
              const cx<R>* __restrict__ x0, double* __restrict__ partial, Geom g, int cb_in,
              double two_a5, double nm_two_a5, double alpha) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[16];
-  const int s = threadIdx.y;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw);  // [2 parities][8][ns][9]
+  const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
+  const int blk = xcd_block(g.nxcd);
+  const int i = blk * ns + il;
+  // slice s holds output parity (s + 1 + cb_in) % 2: set s % 2 of the stage
+  stage_links(lu, U, g, blk, ns, 2, (1 + cb_in) & 1);
+  __syncthreads();
   double nrm = 0.0;
   if (i < g.vh) {
     const int p_out = (s + 1 + cb_in) & 1;
@@ -237,7 +314,8 @@ k_dwf_dslash(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>*
     cx<R> acc[12];
 #pragma unroll
     for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
-    hop4<DAG>(acc, PlainLoader<R>{in + s * slice, g.vh}, U, g, i, p_out);
+    hop4<DAG>(acc, PlainLoader<R>{in + s * slice, g.vh}, lu + ((s & 1) * 8 * ns + il) * 9, ns * 9,
+              g, i, p_out);
     // fifth dimension: the same checkerboard index i in slices s-1 / s+1
     const cx<R>* sm = in + (s > 0 ? (s - 1) : (g.ls - 1)) * slice + i;
     const cx<R>* sp = in + (s < g.ls - 1 ? (s + 1) : 0) * slice + i;
@@ -336,26 +414,54 @@ __device__ __forceinline__ void m5inv_columns(cx<R>* col, int sst, int ns, int n
   }
 }
 
-template <typename R, int DAG, int EPI, bool NORM>
-__global__ void __launch_bounds__(512)
+// FL flags of k_mobius
+enum { FL_NORM = 1, FL_PRE_T = 2, FL_POST_T = 4 };
+// FL_NORM   : |out|^2 partials (dag 0 EPI_SCHUR)
+// FL_PRE_T  : dag 0, `in` already holds (b + c D5) applied to the operand, so
+//             the gather is a plain load
+// FL_POST_T : dag 0 EPI_M5INV, write (b + c D5) M5^-1 (...) so that the next
+//             pass can use FL_PRE_T
+template <typename R>
+__device__ __forceinline__ void apply_t_col(cx<R>* tv, const cx<R>* col_site, int s, int ls,
+                                            const MobArgs& a, int dag) {
+  // (b + c D5) on one s-column held in LDS: dag 0 takes P+ from s-1 and P-
+  // from s+1, dag 1 the reverse (zmobius_dslash_4.C:90-105, :189-202)
+  const cx<R>* h0 = col_site + s * 12;
+  const cx<R>* hm = col_site + (s > 0 ? s - 1 : ls - 1) * 12;
+  const cx<R>* hp = col_site + (s < ls - 1 ? s + 1 : 0) * 12;
+  const R cm = (R)(s > 0 ? a.c : a.mc), cp = (R)(s < ls - 1 ? a.c : a.mc);
+  const R b = (R)a.b;
+#pragma unroll
+  for (int c = 0; c < 12; ++c) {
+    const bool from_minus = (c < 6) == (dag == 0);
+    tv[c] = b * h0[c] + (from_minus ? cm * hm[c] : cp * hp[c]);
+  }
+}
+
+template <typename R, int DAG, int EPI, int FL, int WPE>
+__global__ void __launch_bounds__(512, WPE)
 k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __restrict__ psi,
          const cx<R>* __restrict__ U, double* __restrict__ partial, Geom g, int p_out, MobArgs a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[16];
-  cx<R>* col = reinterpret_cast<cx<R>*>(smem_raw);
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
-  const int sst = g.ls * 12 + 1;  // padded site stride in LDS
-  const int i = blockIdx.x * ns + il;
+  cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw);  // staged links [8][ns][9]
+  cx<R>* col = lu + 8 * ns * 9;                     // s-columns [ns][sst]
+  const int sst = g.ls * 12 + 1;                    // padded site stride in LDS
+  const int blk = xcd_block(g.nxcd);
+  const int i = blk * ns + il;
   const bool act = i < g.vh;
-  const int nvalid = min(ns, g.vh - (int)(blockIdx.x * ns));
+  const int nvalid = min(ns, g.vh - blk * ns);
   const size_t slice = (size_t)12 * g.vh;
   const int ls = g.ls;
 
+  stage_links(lu, U, g, blk, ns, 1, p_out);
+  __syncthreads();
   cx<R> acc[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
   if (act) {
-    if (DAG == 0) {
+    if (DAG == 0 && !(FL & FL_PRE_T)) {
       MobTLoader<R> ld;
       ld.cur = in + s * slice;
       ld.prv = in + (s > 0 ? s - 1 : ls - 1) * slice;
@@ -364,9 +470,9 @@ k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __r
       ld.b = (R)a.b;
       ld.cm = (R)(s > 0 ? a.c : a.mc);
       ld.cp = (R)(s < ls - 1 ? a.c : a.mc);
-      hop4<0>(acc, ld, U, g, i, p_out);
+      hop4<0>(acc, ld, lu + il * 9, ns * 9, g, i, p_out);
     } else {
-      hop4<1>(acc, PlainLoader<R>{in + s * slice, g.vh}, U, g, i, p_out);
+      hop4<DAG>(acc, PlainLoader<R>{in + s * slice, g.vh}, lu + il * 9, ns * 9, g, i, p_out);
     }
   }
   cx<R>* o = out + s * slice + i;
@@ -391,10 +497,10 @@ k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __r
         const cx<R> d5 = (c < 6) ? km * pm[c * g.vh] : kp * pp[c * g.vh];
         const cx<R> v = p0[c * g.vh] + d5 + nkb * acc[c];
         o[c * g.vh] = v;
-        if (NORM) nrm += (double)v.re * v.re + (double)v.im * v.im;
+        if (FL & FL_NORM) nrm += (double)v.re * v.re + (double)v.im * v.im;
       }
     }
-    if (NORM) {
+    if (FL & FL_NORM) {
       const double t = block_sum(nrm, sh);
       if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = t;
     }
@@ -407,15 +513,8 @@ k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __r
 #pragma unroll
     for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = acc[c];
     __syncthreads();
-    const cx<R>* hm = mycol + (s > 0 ? s - 1 : ls - 1) * 12;
-    const cx<R>* hp = mycol + (s < ls - 1 ? s + 1 : 0) * 12;
-    const R cm = (R)(s > 0 ? a.c : a.mc), cp = (R)(s < ls - 1 ? a.c : a.mc);
-    const R b = (R)a.b;
     cx<R> tv[12];
-#pragma unroll
-    for (int c = 0; c < 6; ++c) tv[c] = b * acc[c] + cp * hp[c];  // P+ from s+1
-#pragma unroll
-    for (int c = 6; c < 12; ++c) tv[c] = b * acc[c] + cm * hm[c];  // P- from s-1
+    apply_t_col(tv, mycol, s, ls, a, 1);
     if (EPI == EPI_PLAIN) {
       if (act)
 #pragma unroll
@@ -449,9 +548,16 @@ k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __r
   m5inv_columns<DAG>(col, sst, ns, nvalid, ls, a.m5);
   __syncthreads();
   if (act) {
-    const R sc = (R)(DAG ? a.kappa_b : 1.0);
+    if (DAG == 0 && (FL & FL_POST_T)) {
+      cx<R> tv[12];
+      apply_t_col(tv, mycol, s, ls, a, 0);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) o[c * g.vh] = sc * mycol[s * 12 + c];
+      for (int c = 0; c < 12; ++c) o[c * g.vh] = tv[c];
+    } else {
+      const R sc = (R)(DAG ? a.kappa_b : 1.0);
+#pragma unroll
+      for (int c = 0; c < 12; ++c) o[c * g.vh] = sc * mycol[s * 12 + c];
+    }
   }
 }
 
@@ -464,9 +570,10 @@ k_m5inv(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, Geom g, M5Side k)
   cx<R>* col = reinterpret_cast<cx<R>*>(smem_raw);
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
   const int sst = g.ls * 12 + 1;
-  const int i = blockIdx.x * ns + il;
+  const int blk = xcd_block(g.nxcd);
+  const int i = blk * ns + il;
   const bool act = i < g.vh;
-  const int nvalid = min(ns, g.vh - (int)(blockIdx.x * ns));
+  const int nvalid = min(ns, g.vh - blk * ns);
   const size_t slice = (size_t)12 * g.vh;
   cx<R>* mycol = col + il * sst;
   if (act)
@@ -565,39 +672,49 @@ k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {
     dst[k] = src[k];
 }
 
-// deterministic final sum of per-block partials (one work-group)
-__global__ void __launch_bounds__(256)
+// deterministic final sum of per-block partials (one work-group of 1024
+// threads, four independent accumulators per thread, fixed order)
+__global__ void __launch_bounds__(1024)
 k_reduce_partials(const double* __restrict__ partial, int n, double* __restrict__ out) {
   __shared__ double sh[16];
-  double acc = 0.0;
-  for (int k = threadIdx.x; k < n; k += blockDim.x) acc += partial[k];
-  const double t = block_sum(acc, sh);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int k = threadIdx.x;
+  for (; k + 3 * 1024 < n; k += 4 * 1024) {
+    a0 += partial[k];
+    a1 += partial[k + 1024];
+    a2 += partial[k + 2 * 1024];
+    a3 += partial[k + 3 * 1024];
+  }
+  for (; k < n; k += 1024) a0 += partial[k];
+  const double t = block_sum((a0 + a1) + (a2 + a3), sh);
   if (threadIdx.x == 0) out[0] = t;
 }
 
-// CPS half vector [s][site][24 reals] (FP64, host order) <-> device SoA
+// CPS half vector [s][cb site][24 reals] (FP64, host order) <-> device SoA
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_cps_to_soa(cx<R>* __restrict__ soa, const double* __restrict__ cps, int vh, int ls) {
+k_cps_to_soa(cx<R>* __restrict__ soa, const double* __restrict__ cps, Geom g) {
+  const int vh = g.vh, ls = g.ls;
   const size_t nsite = (size_t)vh * ls;
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < nsite * 12;
        k += (size_t)gridDim.x * blockDim.x) {
     const size_t site = k / 12;  // s*vh + i
     const int c = (int)(k % 12);
-    const size_t s = site / vh, i = site % vh;
+    const size_t s = site / vh, i = site_of_cps(g, (int)(site % vh));
     soa[(s * 12 + c) * vh + i] = cx<R>{(R)cps[k * 2], (R)cps[k * 2 + 1]};
   }
 }
 
 template <typename R>
 __global__ void __launch_bounds__(256)
-k_soa_to_cps(double* __restrict__ cps, const cx<R>* __restrict__ soa, int vh, int ls) {
+k_soa_to_cps(double* __restrict__ cps, const cx<R>* __restrict__ soa, Geom g) {
+  const int vh = g.vh, ls = g.ls;
   const size_t nsite = (size_t)vh * ls;
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < nsite * 12;
        k += (size_t)gridDim.x * blockDim.x) {
     const size_t site = k / 12;
     const int c = (int)(k % 12);
-    const size_t s = site / vh, i = site % vh;
+    const size_t s = site / vh, i = site_of_cps(g, (int)(site % vh));
     const cx<R> v = soa[(s * 12 + c) * vh + i];
     cps[k * 2] = (double)v.re;
     cps[k * 2 + 1] = (double)v.im;
