@@ -25,6 +25,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Time-to-1e-10 solve vs plain FP64 CG (same GPU); MatPcDagMatPc HBM GB/s"
 HBM_PEAK_GBS = 8000.0
+SLOPPY = ("f32", "bf16", "i16")
+KERNELS = {
+    "dwf": ["dwf_hop_cb1", "dwf_hop_cb0_xpay_norm", "dwf_hop_cb1_dag", "dwf_hop_cb0_xpay_dag"],
+    "mobius": ["mobius_hop_m5inv", "mobius_hop_schur_norm", "mobius_hop_m5inv_dag",
+               "mobius_hop_schur_dag"],
+}
 
 WORKLOADS = {
     "mobius_24x24x24x64_ls12": dict(action="mobius", dims=(24, 24, 24, 64), ls=12, mass=0.005,
@@ -69,10 +75,14 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="mobius_24x24x24x64_ls12", choices=sorted(WORKLOADS))
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32"])
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-iter", type=int, default=20000)
+    ap.add_argument("--mixed", nargs="*", default=["f32", "bf16", "i16"],
+                    choices=["f32", "bf16", "i16"],
+                    help="sloppy precisions of the reliable-update CG solves")
+    ap.add_argument("--delta", type=float, default=0.1,
+                    help="reliable update when |r| drops by this factor")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU "
@@ -85,16 +95,15 @@ def main():
     wl = WORKLOADS[args.workload]
     dims, ls, mass = wl["dims"], wl["ls"], wl["mass"]
     vh = int(np.prod(dims)) // 2
-    log(f"[bench] workload {args.workload} precision {args.precision}: generating inputs")
+    log(f"[bench] workload {args.workload}: generating inputs")
     t0 = time.perf_counter()
     U = lat.random_gauge(dims, wl["eps"], 1234)
     src = lat.to_reals(lat.gaussian_spinor((ls, vh), 5678))
     log(f"[bench] inputs ready in {time.perf_counter() - t0:.1f}s; uploading")
     if wl["action"] == "dwf":
-        op = DiracOpDwfGpu(U, dims, ls, m5=1.8, precision=args.precision)
+        op = DiracOpDwfGpu(U, dims, ls, m5=1.8)
     else:
-        op = DiracOpMobiusGpu(U, dims, ls, m5=1.8, b=wl["b"], c=wl["c"],
-                              precision=args.precision)
+        op = DiracOpMobiusGpu(U, dims, ls, m5=1.8, b=wl["b"], c=wl["c"])
     vin, vout = op.vector(src), op.vector()
     bytes_app, flops_app = op.cost()
 
@@ -112,18 +121,49 @@ def main():
     k_gbs = bytes_k[hop] / (ms_k[hop] * 1e-3) / 1e9
     copy_gbs = op.time_copy(2 * 1024 ** 3, 10)
     log(f"[bench] dominant kernel #{hop}: {k_gbs:.0f} GB/s; copy kernel {copy_gbs:.0f} GB/s")
-    prof = traffic_from_profile(args.workload, args.precision)
+    prof = traffic_from_profile(args.workload, "f64")
+
+    sloppy = {}
+    for p in SLOPPY:
+        ms_p, msk_p, byk_p = op.time_MatPcDagMatPc(vout, vin, mass, args.steps, precision=p)
+        b_p, _ = op.cost(p)
+        sloppy[p] = {"ms": round(ms_p, 5), "gbs": round(b_p / (ms_p * 1e-3) / 1e9, 2),
+                     "bytes": b_p, "kernels_ms": [round(float(x), 5) for x in msk_p]}
+        log(f"[bench] {p} MatPcDagMatPc {ms_p:.3f} ms  {sloppy[p]['gbs']:.0f} GB/s "
+            f"(algorithmic bytes in {p})")
 
     solve = None
     if not args.no_solve:
+        arg = CgArg(mass, args.max_iter, 1e-10)
         vsol = op.vector()
         t1 = time.perf_counter()
-        it, tr, st = op.InvCg_dev(vsol, vin, CgArg(mass, args.max_iter, 1e-10))
-        ts = time.perf_counter() - t1
-        solve = {"plain_fp64_cg": {"seconds": round(ts, 4), "iters": it,
-                                   "true_res": tr, "status": st,
-                                   "stop_rsd": 1e-10}}
-        log(f"[bench] plain CG: {it} iters, true res {tr:.3e}, {ts:.2f}s")
+        it, tr, st = op.InvCg_dev(vsol, vin, arg)
+        t_plain = time.perf_counter() - t1
+        x_plain = vsol.download()
+        nx = float(np.linalg.norm(x_plain))
+        solve = {"stop_rsd": 1e-10,
+                 "plain_fp64_cg": {"seconds": round(t_plain, 4), "iters": it, "true_res": tr,
+                                   "status": st}}
+        log(f"[bench] plain CG: {it} iters, true res {tr:.3e}, {t_plain:.2f}s")
+        best = None
+        for p in args.mixed:
+            vsol.upload(np.zeros(op.n))
+            t1 = time.perf_counter()
+            it, tr, st, nrel = op.InvCgMixed_dev(vsol, vin, arg, p, args.delta)
+            ts = time.perf_counter() - t1
+            err = float(np.linalg.norm(vsol.download() - x_plain)) / nx
+            solve[f"mixed_{p}_cg"] = {"seconds": round(ts, 4), "iters": it, "true_res": tr,
+                                      "status": st, "reliable_updates": nrel,
+                                      "delta": args.delta, "rel_err_vs_plain_fp64": err,
+                                      "speedup_vs_plain_fp64": round(t_plain / ts, 3)}
+            log(f"[bench] mixed {p} CG: {it} iters, {nrel} reliable updates, true res "
+                f"{tr:.3e}, |x - x_fp64|/|x_fp64| {err:.2e}, {ts:.2f}s "
+                f"({t_plain / ts:.2f}x)")
+            if st == 0 and tr <= 1e-10 and (best is None or ts < best[1]):
+                best = (p, ts)
+        if best:
+            solve["best"] = {"method": f"mixed_{best[0]}_cg",
+                             "speedup_vs_plain_fp64": round(t_plain / best[1], 3)}
         vsol.free()
 
     cpu = None
@@ -147,21 +187,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "f64" else "f32",
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": args.workload, "lattice": list(dims), "ls": ls,
                    "action": wl["action"], "mass": mass, "m5": 1.8,
-                   "mobius_b": wl.get("b"), "mobius_c": wl.get("c"), "gauge_eps": wl["eps"],
-                   "precision": args.precision},
+                   "mobius_b": wl.get("b"), "mobius_c": wl.get("c"), "gauge_eps": wl["eps"]},
         "gflops": round(gflops, 2),
         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
         "copy_kernel_gbs": round(copy_gbs, 1),
         "kernels_ms": [round(float(x), 5) for x in ms_k],
+        "sloppy_matpcdagmatpc": sloppy,
         "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4),
                      "traffic": None if prof is None else prof.get("traffic_bytes"),
-                     "kernel": ["hop_m5inv", "hop_schur", "hop_m5inv_dag",
-                                "hop_schur_dag"][hop],
+                     "kernel": KERNELS[wl["action"]][hop],
                      "algorithmic_bytes": bytes_k[hop]},
         "cpu_baseline": cpu,
         "solve": solve,

@@ -1,5 +1,6 @@
 // dwf_gpu.hip -- host side of the MI355X domain-wall / Mobius library: the
-// C ABI of include/dwf_gpu.h, device memory, kernel launches and the CG loop.
+// C ABI of include/dwf_gpu.h, device memory, kernel launches, the CG loop and
+// the reliable-update mixed-precision CG.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -19,6 +20,10 @@ using dwfk::cx;
 using dwfk::Geom;
 using dwfk::M5Side;
 using dwfk::MobArgs;
+using dwfk::P32;
+using dwfk::P64;
+using dwfk::PBF16;
+using dwfk::PI16;
 
 namespace {
 
@@ -48,38 +53,80 @@ void set_err(const char* fmt, ...) {
     if (r_ != DWFGPU_OK) return r_;   \
   } while (0)
 
+constexpr int NPREC = 4;  // DWFGPU_F64, F32, BF16, I16
+
+// a spinor half vector on the device in some storage precision
+struct DVec {
+  void* d = nullptr;
+  float* sc = nullptr;  // per-(site, slice) scales of the 16-bit formats
+};
+
+// the operator's fields in one storage precision
+struct PrecFields {
+  bool ready = false;
+  void* U = nullptr;  // links (18 reals in FP64, first two rows otherwise)
+  DVec te, t1;        // opposite-parity temporary, MatPc output inside MdagM
+  DVec work[6];       // solver vectors, allocated on first use
+};
 
 }  // namespace
 
 struct dwfgpu_s {
-  int action = 0, prec = 0;
+  int action = 0, prec = 0;  // prec: storage precision of the API vectors
   int dims[4] = {0, 0, 0, 0};
+  int bc[4] = {0, 0, 0, 0};
   int ls = 0;
   double m5 = 0, a5inv = 1, b = 1, c = 0;
   Geom g{};
-  size_t n = 0;        // reals per half vector
-  size_t elem = 8;     // bytes per stored real
+  size_t n = 0;  // reals per half vector
   hipStream_t st = nullptr;
-  void* U = nullptr;
-  void* te = nullptr;  // opposite-parity temporary
-  void* t1 = nullptr;  // MatPc output inside MatPcDagMatPc
+  PrecFields f[NPREC];
   void* cps_stage = nullptr;  // FP64 buffer in CPS order (host-array API)
-  void* work[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double* partial = nullptr;
   int npartial = 0;
   double* dscal = nullptr;
   double* hscal = nullptr;  // pinned
   int grid_hop = 0;
-  int ns = 16;   // sites per work-group of the hopping kernels (x); y = all slices
-  int wpe = 2;   // minimum waves per SIMD the hopping kernels are compiled for
+  int ns = 16;  // sites per work-group of the hopping kernels (x); y = all slices
+  int max_lds = 65536;
 };
 
 struct dwfgpu_vec_s {
   dwfgpu_s* h = nullptr;
-  void* d = nullptr;
+  DVec v;
 };
 
 namespace {
+
+template <class P>
+constexpr int pidx();
+template <>
+constexpr int pidx<P64>() { return DWFGPU_F64; }
+template <>
+constexpr int pidx<P32>() { return DWFGPU_F32; }
+template <>
+constexpr int pidx<PBF16>() { return DWFGPU_BF16; }
+template <>
+constexpr int pidx<PI16>() { return DWFGPU_I16; }
+
+template <class P>
+dwfk::Fld<P> fld(const DVec& v) {
+  return dwfk::Fld<P>{(typename P::S*)v.d, v.sc};
+}
+template <class P>
+dwfk::CFld<P> cfld(const DVec& v) {
+  return dwfk::CFld<P>{(const typename P::S*)v.d, v.sc};
+}
+
+// bytes of one half vector / of all links (once) in policy P
+template <class P>
+double half_bytes(const dwfgpu_s* h) {
+  return (double)h->n / 2 * sizeof(typename P::S) + (P::kScaled ? 4.0 * h->g.vh * h->ls : 0.0);
+}
+template <class P>
+double link_bytes(const dwfgpu_s* h) {
+  return 8.0 * h->g.vh * P::kLinkElems * sizeof(typename P::S);
+}
 
 // ---------------------------------------------------------------------------
 // coefficients
@@ -161,16 +208,69 @@ MobArgs mob_args(const dwfgpu_s* h, double mass, int dag) {
 }
 
 // LDS of the s-column buffer (M5^-1 and (b + c D5)^dag epilogues)
-size_t col_lds_bytes(const dwfgpu_s* h) { return (size_t)h->ns * (h->ls * 12 + 1) * 2 * h->elem; }
-// LDS of the staged links: nsets parities x 8 directions x ns sites x 9
-size_t link_lds_bytes(const dwfgpu_s* h, int nsets) {
-  return (size_t)nsets * 8 * h->ns * 9 * 2 * h->elem;
+size_t col_lds_bytes(const dwfgpu_s* h, size_t creal) {
+  return (size_t)h->ns * (h->ls * 12 + 1) * 2 * creal;
 }
-// largest LDS request of any kernel of this handle
+// LDS of the staged links: nsets parities x 8 directions x ns sites x 9
+size_t link_lds_bytes(const dwfgpu_s* h, int nsets, size_t creal) {
+  return (size_t)nsets * 8 * h->ns * 9 * 2 * creal;
+}
+// largest LDS request of any kernel of this handle (FP64 compute is the widest)
 size_t max_lds_bytes(const dwfgpu_s* h) {
-  const size_t dwf = link_lds_bytes(h, 2);
-  const size_t mob = link_lds_bytes(h, 1) + col_lds_bytes(h);
-  return h->action == DWFGPU_DWF ? dwf : mob;
+  return h->action == DWFGPU_DWF ? link_lds_bytes(h, 2, 8)
+                                 : link_lds_bytes(h, 1, 8) + col_lds_bytes(h, 8);
+}
+
+// ---------------------------------------------------------------------------
+// device memory
+// ---------------------------------------------------------------------------
+int alloc(dwfgpu_s* h, void** p, size_t bytes) {
+  HIPCHK(hipMalloc(p, bytes));
+  HIPCHK(hipMemsetAsync(*p, 0, bytes, h->st));
+  return DWFGPU_OK;
+}
+
+template <class P>
+int alloc_vec(dwfgpu_s* h, DVec* v) {
+  TRY(alloc(h, &v->d, h->n / 2 * sizeof(typename P::S)));
+  if (P::kScaled) TRY(alloc(h, (void**)&v->sc, (size_t)h->g.vh * h->ls * sizeof(float)));
+  return DWFGPU_OK;
+}
+
+void free_vec(DVec* v) {
+  if (v->d) (void)hipFree(v->d);
+  if (v->sc) (void)hipFree(v->sc);
+  v->d = nullptr;
+  v->sc = nullptr;
+}
+
+dim3 blas_grid() { return dim3(dwfk::BLAS_BLOCKS); }
+dim3 blas_block() { return dim3(dwfk::BLAS_THREADS); }
+
+// the fields of precision P: links derived from the FP64 links on the device
+template <class P>
+int ensure_fields(dwfgpu_s* h) {
+  PrecFields& F = h->f[pidx<P>()];
+  if (F.ready) return DWFGPU_OK;
+  if (pidx<P>() != DWFGPU_F64) {
+    TRY(alloc(h, &F.U, (size_t)link_bytes<P>(h)));
+    hipLaunchKernelGGL((dwfk::k_compress_links<P>), dim3(2048), dim3(256), 0, h->st,
+                       (typename P::S*)F.U, (const cx<double>*)h->f[DWFGPU_F64].U, h->g);
+    HIPCHK(hipGetLastError());
+  }
+  TRY(alloc_vec<P>(h, &F.te));
+  TRY(alloc_vec<P>(h, &F.t1));
+  F.ready = true;
+  return DWFGPU_OK;
+}
+
+template <class P>
+int ensure_work(dwfgpu_s* h, int count) {
+  TRY(ensure_fields<P>(h));
+  PrecFields& F = h->f[pidx<P>()];
+  for (int k = 0; k < count; ++k)
+    if (!F.work[k].d) TRY(alloc_vec<P>(h, &F.work[k]));
+  return DWFGPU_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -186,35 +286,26 @@ int fetch_sum(dwfgpu_s* h, int nparts, double* out) {
   return DWFGPU_OK;
 }
 
-dim3 blas_grid() { return dim3(dwfk::BLAS_BLOCKS); }
-dim3 blas_block() { return dim3(dwfk::BLAS_THREADS); }
-
 // ---------------------------------------------------------------------------
-// operator pieces, templated on the storage real R
+// operator pieces for storage policy P
 // ---------------------------------------------------------------------------
-template <typename R>
+template <class P>
 struct Ops {
-  typedef cx<R> C;
+  typedef typename P::R R;
+  typedef typename P::S S;
+  static PrecFields& F(dwfgpu_s* h) { return h->f[pidx<P>()]; }
 
   // DWF hop kernel: epi 0 -> out = D in ; epi 1 -> out = x0 + alpha D in
-  static int dwf(dwfgpu_s* h, double mass, void* out, const void* in, int cb_in, int dag,
-                 int epi, const void* x0, double alpha, bool norm) {
+  static int dwf(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int cb_in, int dag,
+                 int epi, const DVec& x0, double alpha, bool norm) {
     const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const double two_a5 = 2.0 * h->a5inv, nm = -2.0 * mass * h->a5inv;
-    C* o = (C*)out;
-    const C* i = (const C*)in;
-    const C* u = (const C*)h->U;
-    const C* x = (const C*)x0;
-    const size_t lds = link_lds_bytes(h, 2);
-#define DWF_LAUNCH(DAG, EPI, NORM)                                                              \
-  do {                                                                                          \
-    if (h->wpe >= 4)                                                                            \
-      hipLaunchKernelGGL((dwfk::k_dwf_dslash<R, DAG, EPI, NORM, 4>), grid, block, lds, h->st, o,  \
-                         i, u, x, h->partial, h->g, cb_in, two_a5, nm, alpha);                  \
-    else                                                                                        \
-      hipLaunchKernelGGL((dwfk::k_dwf_dslash<R, DAG, EPI, NORM, 2>), grid, block, lds, h->st, o,  \
-                         i, u, x, h->partial, h->g, cb_in, two_a5, nm, alpha);                  \
-  } while (0)
+    const S* u = (const S*)F(h).U;
+    const size_t lds = link_lds_bytes(h, 2, sizeof(R));
+#define DWF_LAUNCH(DAG, EPI, NORM)                                                           \
+  hipLaunchKernelGGL((dwfk::k_dwf_dslash<P, DAG, EPI, NORM>), grid, block, lds, h->st,       \
+                     fld<P>(out), cfld<P>(in), u, cfld<P>(x0), h->partial, h->g, cb_in, two_a5, \
+                     nm, alpha)
     if (epi == 0) {
       if (dag) DWF_LAUNCH(1, 0, false); else DWF_LAUNCH(0, 0, false);
     } else if (norm) {
@@ -228,25 +319,16 @@ struct Ops {
   }
 
   // Mobius hop kernel; fl: dwfk::FL_* flags
-  static int mob(dwfgpu_s* h, double mass, void* out, const void* in, const void* psi, int p_out,
-                 int dag, int epi, int fl) {
+  static int mob(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, const DVec& psi,
+                 int p_out, int dag, int epi, int fl) {
     const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const MobArgs a = mob_args(h, mass, dag);
-    C* o = (C*)out;
-    const C* i = (const C*)in;
-    const C* p = (const C*)psi;
-    const C* u = (const C*)h->U;
+    const S* u = (const S*)F(h).U;
     const bool col = dag || epi == dwfk::EPI_M5INV;
-    const size_t lds = link_lds_bytes(h, 1) + (col ? col_lds_bytes(h) : 0);
-#define MOB_LAUNCH(DAG, EPI, FL)                                                               \
-  do {                                                                                         \
-    if (h->wpe >= 4)                                                                           \
-      hipLaunchKernelGGL((dwfk::k_mobius<R, DAG, EPI, FL, 4>), grid, block, lds, h->st, o, i, p, \
-                         u, h->partial, h->g, p_out, a);                                       \
-    else                                                                                       \
-      hipLaunchKernelGGL((dwfk::k_mobius<R, DAG, EPI, FL, 2>), grid, block, lds, h->st, o, i, p, \
-                         u, h->partial, h->g, p_out, a);                                       \
-  } while (0)
+    const size_t lds = link_lds_bytes(h, 1, sizeof(R)) + (col ? col_lds_bytes(h, sizeof(R)) : 0);
+#define MOB_LAUNCH(DAG, EPI, FL)                                                              \
+  hipLaunchKernelGGL((dwfk::k_mobius<P, DAG, EPI, FL>), grid, block, lds, h->st, fld<P>(out), \
+                     cfld<P>(in), cfld<P>(psi), u, h->partial, h->g, p_out, a)
     using dwfk::FL_NORM;
     using dwfk::FL_POST_T;
     using dwfk::FL_PRE_T;
@@ -273,99 +355,117 @@ struct Ops {
   }
 
   // first / second half of MatPc (dag 0) or MatPcDag (dag 1): the first writes
-  // h->te (for dag 0 already multiplied by (b + c D5), consumed by the second)
-  static int mob_first(dwfgpu_s* h, double mass, void* te, const void* in, int dag) {
-    return mob(h, mass, te, in, nullptr, 0, dag, dwfk::EPI_M5INV, dag ? 0 : dwfk::FL_POST_T);
+  // te (for dag 0 already multiplied by (b + c D5), consumed by the second)
+  static int mob_first(dwfgpu_s* h, double mass, const DVec& te, const DVec& in, int dag) {
+    return mob(h, mass, te, in, DVec(), 0, dag, dwfk::EPI_M5INV, dag ? 0 : dwfk::FL_POST_T);
   }
-  static int mob_second(dwfgpu_s* h, double mass, void* out, const void* te, const void* in,
-                        int dag, bool norm) {
+  static int mob_second(dwfgpu_s* h, double mass, const DVec& out, const DVec& te,
+                        const DVec& in, int dag, bool norm) {
     const int fl = (dag ? 0 : dwfk::FL_PRE_T) | (norm && !dag ? dwfk::FL_NORM : 0);
     return mob(h, mass, out, te, in, 1, dag, dwfk::EPI_SCHUR, fl);
   }
 
-  static int m5inv(dwfgpu_s* h, double mass, void* out, const void* in, int dag) {
+  static int m5inv(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int dag) {
     const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const M5Side k = m5_side(h, mass, dag);
+    const size_t lds = col_lds_bytes(h, sizeof(R));
     if (dag)
-      hipLaunchKernelGGL((dwfk::k_m5inv<R, 1>), grid, block, col_lds_bytes(h), h->st, (C*)out,
-                         (const C*)in, h->g, k);
+      hipLaunchKernelGGL((dwfk::k_m5inv<P, 1>), grid, block, lds, h->st, fld<P>(out), cfld<P>(in),
+                         h->g, k);
     else
-      hipLaunchKernelGGL((dwfk::k_m5inv<R, 0>), grid, block, col_lds_bytes(h), h->st, (C*)out,
-                         (const C*)in, h->g, k);
+      hipLaunchKernelGGL((dwfk::k_m5inv<P, 0>), grid, block, lds, h->st, fld<P>(out), cfld<P>(in),
+                         h->g, k);
     HIPCHK(hipGetLastError());
     return DWFGPU_OK;
   }
 
   // MatPc / MatPcDag: out = M in (out on the CG parity).  norm -> partials.
-  static int matpc(dwfgpu_s* h, double mass, void* out, const void* in, int dag, bool norm) {
+  static int matpc(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int dag,
+                   bool norm) {
+    const DVec& te = F(h).te;
     if (h->action == DWFGPU_DWF) {
       const double mk2 = -dwf_kappa(h) * dwf_kappa(h);
-      TRY(dwf(h, mass, h->te, in, 1, dag, 0, nullptr, 0.0, false));
-      TRY(dwf(h, mass, out, h->te, 0, dag, 1, in, mk2, norm));
+      TRY(dwf(h, mass, te, in, 1, dag, 0, DVec(), 0.0, false));
+      TRY(dwf(h, mass, out, te, 0, dag, 1, in, mk2, norm));
     } else {
-      TRY(mob_first(h, mass, h->te, in, dag));
-      TRY(mob_second(h, mass, out, h->te, in, dag, norm));
+      TRY(mob_first(h, mass, te, in, dag));
+      TRY(mob_second(h, mass, out, te, in, dag, norm));
     }
     return DWFGPU_OK;
   }
 
-  static int mdagm(dwfgpu_s* h, double mass, void* out, const void* in, double* dot) {
-    TRY(matpc(h, mass, h->t1, in, 0, dot != nullptr));
+  // MatPcDagMatPc; *dot (if requested) = |MatPc in|^2 (dwf_mdagm.C:49-55)
+  static int mdagm(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, double* dot) {
+    const DVec& t1 = F(h).t1;
+    TRY(matpc(h, mass, t1, in, 0, dot != nullptr));
     if (dot) TRY(fetch_sum(h, h->grid_hop, dot));
-    TRY(matpc(h, mass, out, h->t1, 1, false));
+    TRY(matpc(h, mass, out, t1, 1, false));
     return DWFGPU_OK;
   }
 
-  static int dslash(dwfgpu_s* h, double mass, void* out, const void* in, int cb, int dag) {
-    if (h->action == DWFGPU_DWF) return dwf(h, mass, out, in, cb, dag, 0, nullptr, 0.0, false);
-    return mob(h, mass, out, in, nullptr, 1 - cb, dag, dwfk::EPI_PLAIN, 0);
+  static int dslash(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int cb, int dag) {
+    if (h->action == DWFGPU_DWF) return dwf(h, mass, out, in, cb, dag, 0, DVec(), 0.0, false);
+    return mob(h, mass, out, in, DVec(), 1 - cb, dag, dwfk::EPI_PLAIN, 0);
   }
 
-  static int norm(dwfgpu_s* h, const void* x, double* out) {
-    hipLaunchKernelGGL((dwfk::k_norm<R>), blas_grid(), blas_block(), 0, h->st, (const R*)x, h->n,
+  static int norm(dwfgpu_s* h, const DVec& x, double* out) {
+    hipLaunchKernelGGL((dwfk::k_norm<P>), blas_grid(), blas_block(), 0, h->st, cfld<P>(x), h->g,
                        h->partial);
     HIPCHK(hipGetLastError());
     return fetch_sum(h, dwfk::BLAS_BLOCKS, out);
   }
 
-  static int axpy_to(dwfgpu_s* h, void* z, const void* x, const void* y, double a) {
-    hipLaunchKernelGGL((dwfk::k_axpy_to<R, false>), blas_grid(), blas_block(), 0, h->st, (R*)z,
-                       (const R*)x, (const R*)y, a, h->n, h->partial);
+  // z = a x + y (with |z|^2 partials when norm)
+  static int axpy_to(dwfgpu_s* h, const DVec& z, const DVec& x, const DVec& y, double a,
+                     bool norm) {
+    if (norm)
+      hipLaunchKernelGGL((dwfk::k_axpy_to<P, true>), blas_grid(), blas_block(), 0, h->st, fld<P>(z),
+                         cfld<P>(x), cfld<P>(y), a, h->g, h->partial);
+    else
+      hipLaunchKernelGGL((dwfk::k_axpy_to<P, false>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P>(z), cfld<P>(x), cfld<P>(y), a, h->g, h->partial);
     HIPCHK(hipGetLastError());
     return DWFGPU_OK;
   }
 
-  static int axpby(dwfgpu_s* h, void* y, const void* x, double a, double b) {
-    hipLaunchKernelGGL((dwfk::k_axpby<R>), blas_grid(), blas_block(), 0, h->st, (R*)y,
-                       (const R*)x, a, b, h->n);
+  // y = a x + b y
+  static int axpby(dwfgpu_s* h, const DVec& y, const DVec& x, double a, double b) {
+    hipLaunchKernelGGL((dwfk::k_axpby<P>), blas_grid(), blas_block(), 0, h->st, fld<P>(y),
+                       cfld<P>(x), a, b, h->g);
     HIPCHK(hipGetLastError());
     return DWFGPU_OK;
   }
 
-  static int scale(dwfgpu_s* h, void* x, double a) {
-    hipLaunchKernelGGL((dwfk::k_scale<R>), blas_grid(), blas_block(), 0, h->st, (R*)x, a, h->n);
+  static int scale(dwfgpu_s* h, const DVec& x, double a) {
+    hipLaunchKernelGGL((dwfk::k_scale<P>), blas_grid(), blas_block(), 0, h->st, fld<P>(x), a, h->g);
     HIPCHK(hipGetLastError());
+    return DWFGPU_OK;
+  }
+
+  static int copy(dwfgpu_s* h, const DVec& dst, const DVec& src) {
+    HIPCHK(hipMemcpyAsync(dst.d, src.d, h->n / 2 * sizeof(S), hipMemcpyDeviceToDevice, h->st));
+    if (P::kScaled)
+      HIPCHK(hipMemcpyAsync(dst.sc, src.sc, (size_t)h->g.vh * h->ls * sizeof(float),
+                            hipMemcpyDeviceToDevice, h->st));
     return DWFGPU_OK;
   }
 
   // DiracOp::InvCgShift, shift = NULL (inv_cg.C:84-448)
-  static int invcg(dwfgpu_s* h, const dwfgpu_cg_arg* arg, void* sol, const void* src,
+  static int invcg(dwfgpu_s* h, const dwfgpu_cg_arg* arg, const DVec& sol, const DVec& src,
                    double* true_res, int* iters) {
-    void* res = h->work[0];
-    void* dir = h->work[1];
-    void* mmp = h->work[2];
+    TRY(ensure_work<P>(h, 3));
+    const DVec& res = F(h).work[0];
+    const DVec& dir = F(h).work[1];
+    const DVec& mmp = F(h).work[2];
     const double mass = arg->mass;
     double src_norm_sq;
     TRY(norm(h, src, &src_norm_sq));
     const double stp_cnd = src_norm_sq * arg->stop_rsd * arg->stop_rsd;
     TRY(mdagm(h, mass, mmp, sol, nullptr));
-    // res = src - mmp ; dir = res ; rr = |res|^2
-    hipLaunchKernelGGL((dwfk::k_axpy_to<R, true>), blas_grid(), blas_block(), 0, h->st, (R*)res,
-                       (const R*)mmp, (const R*)src, -1.0, h->n, h->partial);
-    HIPCHK(hipGetLastError());
+    TRY(axpy_to(h, res, mmp, src, -1.0, true));  // res = src - mmp
     double rr;
     TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
-    HIPCHK(hipMemcpyAsync(dir, res, h->n * h->elem, hipMemcpyDeviceToDevice, h->st));
+    TRY(copy(h, dir, res));
     int itr = 0;
     int max_itr = arg->max_num_iter - 1;
     if (rr <= stp_cnd) max_itr = 0;
@@ -380,8 +480,8 @@ struct Ops {
         break;
       }
       const double a = rr_prv / d;
-      hipLaunchKernelGGL((dwfk::k_cg_update<R>), blas_grid(), blas_block(), 0, h->st, (R*)sol,
-                         (R*)res, (const R*)dir, (const R*)mmp, a, h->n, h->partial);
+      hipLaunchKernelGGL((dwfk::k_cg_update<P, P>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P>(sol), fld<P>(res), cfld<P>(dir), cfld<P>(mmp), a, h->g, h->partial);
       HIPCHK(hipGetLastError());
       TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
       if (rr <= stp_cnd) break;
@@ -390,9 +490,7 @@ struct Ops {
     }
     // true residual |src - A sol| / |src|  (inv_cg.C:403-415)
     TRY(mdagm(h, mass, mmp, sol, nullptr));
-    hipLaunchKernelGGL((dwfk::k_axpy_to<R, true>), blas_grid(), blas_block(), 0, h->st, (R*)res,
-                       (const R*)mmp, (const R*)src, -1.0, h->n, h->partial);
-    HIPCHK(hipGetLastError());
+    TRY(axpy_to(h, res, mmp, src, -1.0, true));
     double tr;
     TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &tr));
     if (true_res) *true_res = std::sqrt(tr / src_norm_sq);
@@ -402,13 +500,15 @@ struct Ops {
   }
 
   // DiracOpDwf::MatInv (d_op_dwf.C:316-507) / DiracOpMobius::MatInv ORIG
-  // (d_op_mobius.C:407-696).  fin/fout: the two device halves in CPS order of
-  // the full vector (first, second).
-  static int matinv(dwfgpu_s* h, const dwfgpu_cg_arg* arg, void* out_first, void* out_second,
-                    const void* in_first, const void* in_second, double* true_res, int* iters) {
+  // (d_op_mobius.C:407-696) on the two device halves (first, second) of the
+  // full vectors in CPS order.
+  static int matinv(dwfgpu_s* h, const dwfgpu_cg_arg* arg, const DVec& out_first,
+                    const DVec& out_second, const DVec& in_first, const DVec& in_second,
+                    double* true_res, int* iters) {
+    TRY(ensure_work<P>(h, 6));
     const double mass = arg->mass;
-    void* temp = h->work[3];
-    void* src = h->work[4];
+    const DVec& temp = F(h).work[3];
+    const DVec& src = F(h).work[4];
     int st;
     if (h->action == DWFGPU_DWF) {
       const double kappa = dwf_kappa(h);
@@ -421,14 +521,14 @@ struct Ops {
     } else {
       // [even | odd]
       const double kb = mob_kappa_b(h);
-      void* temp2 = h->work[5];
+      const DVec& temp2 = F(h).work[5];
       TRY(m5inv(h, mass, temp, in_first, 0));
-      TRY(mob(h, mass, temp2, temp, nullptr, 1, 0, dwfk::EPI_PLAIN, 0));
-      TRY(axpy_to(h, temp, temp2, in_second, kb));
+      TRY(mob(h, mass, temp2, temp, DVec(), 1, 0, dwfk::EPI_PLAIN, 0));
+      TRY(axpy_to(h, temp, temp2, in_second, kb, false));
       TRY(matpc(h, mass, src, temp, 1, false));
       st = invcg(h, arg, out_second, src, true_res, iters);
       if (st != DWFGPU_OK && st != DWFGPU_MAXITER) return st;
-      TRY(mob(h, mass, temp, out_second, nullptr, 0, 0, dwfk::EPI_PLAIN, 0));
+      TRY(mob(h, mass, temp, out_second, DVec(), 0, 0, dwfk::EPI_PLAIN, 0));
       TRY(scale(h, temp, kb));
       TRY(m5inv(h, mass, out_first, temp, 0));
       TRY(m5inv(h, mass, temp, in_first, 0));
@@ -439,18 +539,18 @@ struct Ops {
     return st;
   }
 
-  static int to_device(dwfgpu_s* h, void* dst, const double* host) {
+  static int to_device(dwfgpu_s* h, const DVec& dst, const double* host) {
     HIPCHK(hipMemcpyAsync(h->cps_stage, host, h->n * sizeof(double), hipMemcpyHostToDevice,
                           h->st));
-    hipLaunchKernelGGL((dwfk::k_cps_to_soa<R>), dim3(2048), dim3(256), 0, h->st, (C*)dst,
+    hipLaunchKernelGGL((dwfk::k_cps_to_field<P>), dim3(2048), dim3(256), 0, h->st, fld<P>(dst),
                        (const double*)h->cps_stage, h->g);
     HIPCHK(hipGetLastError());
     return DWFGPU_OK;
   }
 
-  static int to_host(dwfgpu_s* h, double* host, const void* src) {
-    hipLaunchKernelGGL((dwfk::k_soa_to_cps<R>), dim3(2048), dim3(256), 0, h->st,
-                       (double*)h->cps_stage, (const C*)src, h->g);
+  static int to_host(dwfgpu_s* h, double* host, const DVec& src) {
+    hipLaunchKernelGGL((dwfk::k_field_to_cps<P>), dim3(2048), dim3(256), 0, h->st,
+                       (double*)h->cps_stage, cfld<P>(src), h->g);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(host, h->cps_stage, h->n * sizeof(double), hipMemcpyDeviceToHost,
                           h->st));
@@ -459,26 +559,103 @@ struct Ops {
   }
 };
 
-#define DISPATCH(h, call) ((h)->prec == DWFGPU_F64 ? Ops<double>::call : Ops<float>::call)
-
-int alloc(dwfgpu_s* h, void** p, size_t bytes) {
-  HIPCHK(hipMalloc(p, bytes));
-  HIPCHK(hipMemsetAsync(*p, 0, bytes, h->st));
-  return DWFGPU_OK;
+// FP64-reliable-update CG with the bulk of MatPcDagMatPc in sloppy policy PS.
+// High precision (FP64): sol, src, the true residual r and the solution
+// increments; sloppy: r_s, p_s, A p_s; the running correction x_s is FP32.
+// A reliable update (x += x_s, r = src - A x in FP64, r_s = r) happens when
+// |r_s| falls below delta times |r| at the previous update, and whenever the
+// sloppy residual meets the stop condition; the solve ends only when the
+// FP64 true residual meets it (stop rule of inv_cg.C:174,354).
+template <class PS>
+int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec& sol,
+                const DVec& src, double* true_res, int* iters, int* nreliable) {
+  typedef Ops<P64> H;
+  typedef Ops<PS> L;
+  TRY(ensure_work<P64>(h, 2));
+  TRY(ensure_work<PS>(h, 3));
+  TRY(ensure_work<P32>(h, 4));
+  const DVec& r = h->f[DWFGPU_F64].work[0];
+  const DVec& ax = h->f[DWFGPU_F64].work[1];
+  const DVec& rs = h->f[pidx<PS>()].work[0];
+  const DVec& ps = h->f[pidx<PS>()].work[1];
+  const DVec& aps = h->f[pidx<PS>()].work[2];
+  const DVec& xs = h->f[DWFGPU_F32].work[3];
+  const double mass = arg->mass;
+  double src_norm_sq;
+  TRY(H::norm(h, src, &src_norm_sq));
+  const double stp_cnd = src_norm_sq * arg->stop_rsd * arg->stop_rsd;
+  // r = src - A sol ; r_s = p_s = r ; x_s = 0
+  TRY(H::mdagm(h, mass, ax, sol, nullptr));
+  TRY(H::axpy_to(h, r, ax, src, -1.0, true));
+  double rr;
+  TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
+  hipLaunchKernelGGL((dwfk::k_convert<PS, P64>), blas_grid(), blas_block(), 0, h->st, fld<PS>(rs),
+                     cfld<P64>(r), h->g);
+  HIPCHK(hipGetLastError());
+  TRY(L::copy(h, ps, rs));
+  HIPCHK(hipMemsetAsync(xs.d, 0, h->n / 2 * sizeof(cx<float>), h->st));
+  double rr_update = rr;  // |r|^2 at the last reliable update
+  int itr = 0, nrel = 0, status = DWFGPU_OK;
+  const int max_itr = rr <= stp_cnd ? 0 : arg->max_num_iter - 1;
+  bool converged = rr <= stp_cnd;
+  while (!converged && itr < max_itr) {
+    itr = itr + 1;
+    double d;
+    TRY(L::mdagm(h, mass, aps, ps, &d));
+    if (d == 0.0) {
+      status = DWFGPU_BREAKDOWN;
+      break;
+    }
+    const double a = rr / d;
+    hipLaunchKernelGGL((dwfk::k_cg_update<PS, P32>), blas_grid(), blas_block(), 0, h->st,
+                       fld<P32>(xs), fld<PS>(rs), cfld<PS>(ps), cfld<PS>(aps), a, h->g, h->partial);
+    HIPCHK(hipGetLastError());
+    double rr_new;
+    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr_new));
+    if (rr_new <= stp_cnd || rr_new < delta * delta * rr_update) {
+      // reliable update in FP64
+      hipLaunchKernelGGL((dwfk::k_accumulate<P32>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P64>(sol), fld<P32>(xs), h->g);
+      HIPCHK(hipGetLastError());
+      TRY(H::mdagm(h, mass, ax, sol, nullptr));
+      TRY(H::axpy_to(h, r, ax, src, -1.0, true));
+      TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr_new));
+      hipLaunchKernelGGL((dwfk::k_convert<PS, P64>), blas_grid(), blas_block(), 0, h->st,
+                         fld<PS>(rs), cfld<P64>(r), h->g);
+      HIPCHK(hipGetLastError());
+      rr_update = rr_new;
+      ++nrel;
+      if (rr_new <= stp_cnd) {
+        rr = rr_new;
+        converged = true;
+        break;
+      }
+    }
+    const double b = rr_new / rr;
+    TRY(L::axpby(h, ps, rs, 1.0, b));  // p_s = r_s + b p_s
+    rr = rr_new;
+  }
+  if (!converged && status == DWFGPU_OK) {
+    // fold the pending correction in and measure the true residual
+    hipLaunchKernelGGL((dwfk::k_accumulate<P32>), blas_grid(), blas_block(), 0, h->st,
+                       fld<P64>(sol), fld<P32>(xs), h->g);
+    HIPCHK(hipGetLastError());
+    TRY(H::mdagm(h, mass, ax, sol, nullptr));
+    TRY(H::axpy_to(h, r, ax, src, -1.0, true));
+    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
+    status = DWFGPU_MAXITER;
+  }
+  if (true_res) *true_res = std::sqrt(rr / src_norm_sq);
+  if (iters) *iters = itr + 1;
+  if (nreliable) *nreliable = nrel;
+  return status;
 }
 
-int ensure_work(dwfgpu_s* h) {
-  for (int k = 0; k < 6; ++k)
-    if (!h->work[k]) TRY(alloc(h, &h->work[k], h->n * h->elem));
-  return DWFGPU_OK;
-}
-
-// CANONICAL host gauge -> device layout with BondCond signs
-template <typename R>
-int upload_gauge(dwfgpu_s* h, const double* uc, const int bc[4]) {
+// CANONICAL host gauge -> FP64 device layout with BondCond signs
+int upload_gauge(dwfgpu_s* h, const double* uc) {
   const int lx = h->dims[0], ly = h->dims[1], lz = h->dims[2], lt = h->dims[3];
   const size_t vh = h->g.vh;
-  std::vector<R> dev((size_t)4 * 2 * 9 * vh * 2);
+  std::vector<double> dev((size_t)4 * 2 * 9 * vh * 2);
   for (int t = 0; t < lt; ++t)
     for (int z = 0; z < lz; ++z)
       for (int y = 0; y < ly; ++y)
@@ -488,33 +665,37 @@ int upload_gauge(dwfgpu_s* h, const double* uc, const int bc[4]) {
           const size_t cbs = dwfk::site_index(h->g, x / 2, y, z, t);
           const int pos[4] = {x, y, z, t};
           for (int mu = 0; mu < 4; ++mu) {
-            const double sg = (bc[mu] && pos[mu] == h->dims[mu] - 1) ? -1.0 : 1.0;
+            const double sg = (h->bc[mu] && pos[mu] == h->dims[mu] - 1) ? -1.0 : 1.0;
             const double* src = uc + (site * 4 + mu) * 18;
             for (int e = 0; e < 9; ++e) {
               const size_t o = (((size_t)mu * 2 + par) * 9 + e) * vh + cbs;
-              dev[2 * o] = (R)(sg * src[2 * e]);
-              dev[2 * o + 1] = (R)(sg * src[2 * e + 1]);
+              dev[2 * o] = sg * src[2 * e];
+              dev[2 * o + 1] = sg * src[2 * e + 1];
             }
           }
         }
-  HIPCHK(hipMalloc(&h->U, dev.size() * sizeof(R)));
-  HIPCHK(hipMemcpy(h->U, dev.data(), dev.size() * sizeof(R), hipMemcpyHostToDevice));
+  void** U = &h->f[DWFGPU_F64].U;
+  HIPCHK(hipMalloc(U, dev.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(*U, dev.data(), dev.size() * sizeof(double), hipMemcpyHostToDevice));
   return DWFGPU_OK;
 }
 
 // Events around each kernel of MatPcDagMatPc (the 4 hopping passes and the
 // norm's final reduction); they are recorded on the stream the kernels run on.
-template <typename R>
-int time_mdagm(dwfgpu_s* h, double mass, void* out, const void* in, int reps, double* ms_app,
-               double* ms_kernel, double* bytes_kernel, int* nkernel) {
-  typedef Ops<R> O;
+template <class P>
+int time_mdagm(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int reps,
+               double* ms_app, double* ms_kernel, double* bytes_kernel, int* nkernel) {
+  typedef Ops<P> O;
+  TRY(ensure_fields<P>(h));
+  const DVec& te = h->f[pidx<P>()].te;
+  const DVec& t1 = h->f[pidx<P>()].t1;
   const int nk = 5;
   std::vector<hipEvent_t> ev((size_t)reps * nk + 1, nullptr);
   int st = DWFGPU_OK;
   for (auto& e : ev)
     if (hipEventCreate(&e) != hipSuccess) st = DWFGPU_HIP_ERROR;
-  const double half = (double)h->n * h->elem;
-  const double links = 72.0 * 2.0 * h->g.vh * h->elem;
+  const double half = half_bytes<P>(h);
+  const double links = link_bytes<P>(h);
   const double kb[nk] = {2 * half + links, 3 * half + links, 2 * half + links, 3 * half + links, 0};
   const bool dwf = h->action == DWFGPU_DWF;
   const double mk2 = dwf ? -dwf_kappa(h) * dwf_kappa(h) : 0.0;
@@ -524,17 +705,17 @@ int time_mdagm(dwfgpu_s* h, double mass, void* out, const void* in, int reps, do
   for (int r = 0; r < reps && !st; ++r) {
     const size_t b = (size_t)r * nk;
     rec(b + 0);
-    if (!st) st = dwf ? O::dwf(h, mass, h->te, in, 1, 0, 0, nullptr, 0, false)
-                      : O::mob_first(h, mass, h->te, in, 0);
+    if (!st) st = dwf ? O::dwf(h, mass, te, in, 1, 0, 0, DVec(), 0, false)
+                      : O::mob_first(h, mass, te, in, 0);
     rec(b + 1);
-    if (!st) st = dwf ? O::dwf(h, mass, h->t1, h->te, 0, 0, 1, in, mk2, true)
-                      : O::mob_second(h, mass, h->t1, h->te, in, 0, true);
+    if (!st) st = dwf ? O::dwf(h, mass, t1, te, 0, 0, 1, in, mk2, true)
+                      : O::mob_second(h, mass, t1, te, in, 0, true);
     rec(b + 2);
-    if (!st) st = dwf ? O::dwf(h, mass, h->te, h->t1, 1, 1, 0, nullptr, 0, false)
-                      : O::mob_first(h, mass, h->te, h->t1, 1);
+    if (!st) st = dwf ? O::dwf(h, mass, te, t1, 1, 1, 0, DVec(), 0, false)
+                      : O::mob_first(h, mass, te, t1, 1);
     rec(b + 3);
-    if (!st) st = dwf ? O::dwf(h, mass, out, h->te, 0, 1, 1, h->t1, mk2, false)
-                      : O::mob_second(h, mass, out, h->te, h->t1, 1, false);
+    if (!st) st = dwf ? O::dwf(h, mass, out, te, 0, 1, 1, t1, mk2, false)
+                      : O::mob_second(h, mass, out, te, t1, 1, false);
     rec(b + 4);
     if (!st) {
       hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(1024), 0, h->st, h->partial,
@@ -570,13 +751,46 @@ int time_mdagm(dwfgpu_s* h, double mass, void* out, const void* in, int reps, do
 }
 
 void free_all(dwfgpu_s* h) {
-  void* ptrs[] = {h->U, h->te, h->t1, h->cps_stage, h->partial, h->dscal,
-                  h->work[0], h->work[1], h->work[2], h->work[3], h->work[4], h->work[5]};
+  for (int p = 0; p < NPREC; ++p) {
+    PrecFields& F = h->f[p];
+    if (F.U) (void)hipFree(F.U);
+    F.U = nullptr;
+    free_vec(&F.te);
+    free_vec(&F.t1);
+    for (auto& w : F.work) free_vec(&w);
+    F.ready = false;
+  }
+  void* ptrs[] = {h->cps_stage, h->partial, h->dscal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->hscal) (void)hipHostFree(h->hscal);
   if (h->st) (void)hipStreamDestroy(h->st);
 }
+
+// dispatch on the API precision of the handle
+#define DISPATCH(h, call)                                \
+  ((h)->prec == DWFGPU_F64    ? Ops<P64>::call           \
+   : (h)->prec == DWFGPU_F32  ? Ops<P32>::call           \
+   : (h)->prec == DWFGPU_BF16 ? Ops<PBF16>::call         \
+                              : Ops<PI16>::call)
+
+int ensure_api(dwfgpu_s* h, int nwork) {
+  switch (h->prec) {
+    case DWFGPU_F64: return ensure_work<P64>(h, nwork);
+    case DWFGPU_F32: return ensure_work<P32>(h, nwork);
+    case DWFGPU_BF16: return ensure_work<PBF16>(h, nwork);
+    default: return ensure_work<PI16>(h, nwork);
+  }
+}
+int alloc_api_vec(dwfgpu_s* h, DVec* v) {
+  switch (h->prec) {
+    case DWFGPU_F64: return alloc_vec<P64>(h, v);
+    case DWFGPU_F32: return alloc_vec<P32>(h, v);
+    case DWFGPU_BF16: return alloc_vec<PBF16>(h, v);
+    default: return alloc_vec<PI16>(h, v);
+  }
+}
+const DVec& api_work(dwfgpu_s* h, int k) { return h->f[h->prec].work[k]; }
 
 }  // namespace
 
@@ -607,15 +821,17 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
     set_err("dwfgpu_create: unknown action %d", action);
     return DWFGPU_BAD_ARG;
   }
-  if (precision != DWFGPU_F64 && precision != DWFGPU_F32) {
+  if (precision < DWFGPU_F64 || precision > DWFGPU_I16) {
     set_err("dwfgpu_create: unknown precision %d", precision);
     return DWFGPU_BAD_ARG;
   }
   dwfgpu_s* h = new dwfgpu_s;
   h->action = action;
   h->prec = precision;
-  h->elem = precision == DWFGPU_F64 ? 8 : 4;
-  for (int mu = 0; mu < 4; ++mu) h->dims[mu] = dims[mu];
+  for (int mu = 0; mu < 4; ++mu) {
+    h->dims[mu] = dims[mu];
+    h->bc[mu] = bc[mu] ? 1 : 0;
+  }
   h->ls = ls;
   h->m5 = m5;
   h->a5inv = a5inv;
@@ -627,36 +843,36 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
     set_err("dwfgpu_create: local volume too large");
     return DWFGPU_BAD_ARG;
   }
-  // t tiles of 4 slices when possible; 8 XCDs on gfx950 (a different count
-  // only changes the work-group order, not the result)
+  // t and z tiles of 4 slices when possible; 8 XCDs on gfx950 (a different
+  // count only changes the work-group order, not the result)
   const int bt = dims[3] % 4 == 0 ? 4 : 2;
   const int bz = dims[2] % 4 == 0 ? 4 : 2;
-  h->g = Geom{dims[0], dims[1], dims[2], dims[3], dims[0] / 2, (int)(v4 / 2), ls, bt, bz, 8};
+  const int bcmask = h->bc[0] | h->bc[1] << 1 | h->bc[2] << 2 | h->bc[3] << 3;
+  h->g = Geom{dims[0], dims[1], dims[2], dims[3], dims[0] / 2, (int)(v4 / 2), ls, bt, bz, 8,
+              bcmask};
   h->n = (size_t)24 * (v4 / 2) * ls;
-  int dev = 0, max_lds = 0;
+  int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+      hipDeviceGetAttribute(&h->max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
           hipSuccess) {
     set_err("dwfgpu_create: no HIP device");
     delete h;
     return DWFGPU_HIP_ERROR;
   }
-  // launch shape of the hopping kernels; DWFGPU_NS / DWFGPU_WPE override
-  // the defaults (sites per work-group, minimum waves per SIMD).  Without an
-  // override, fewer sites per work-group are taken when the LDS stage of a
-  // long fifth dimension would not fit.
+  // sites per work-group of the hopping kernels; DWFGPU_NS overrides the
+  // default.  Without an override, fewer sites per work-group are taken when
+  // the LDS stage of a long fifth dimension would not fit.
   const char* env_ns = getenv("DWFGPU_NS");
   if (env_ns) h->ns = atoi(env_ns);
-  if (const char* e = getenv("DWFGPU_WPE")) h->wpe = atoi(e);
-  while (!env_ns && h->ns > 4 && max_lds_bytes(h) > (size_t)max_lds) h->ns /= 2;
-  if (h->ns < 1 || h->ns * ls > 512 || (h->wpe != 2 && h->wpe != 4)) {
-    set_err("dwfgpu_create: bad launch shape ns=%d (ns*Ls <= 512) wpe=%d (2 or 4)", h->ns, h->wpe);
+  while (!env_ns && h->ns > 4 && max_lds_bytes(h) > (size_t)h->max_lds) h->ns /= 2;
+  if (h->ns < 1 || h->ns * ls > 512) {
+    set_err("dwfgpu_create: bad launch shape ns=%d (ns*Ls <= 512)", h->ns);
     delete h;
     return DWFGPU_BAD_ARG;
   }
-  if (max_lds_bytes(h) > (size_t)max_lds) {
+  if (max_lds_bytes(h) > (size_t)h->max_lds) {
     set_err("dwfgpu_create: Ls=%d needs %zu bytes of LDS per work-group (limit %d)", ls,
-            max_lds_bytes(h), max_lds);
+            max_lds_bytes(h), h->max_lds);
     delete h;
     return DWFGPU_BAD_ARG;
   }
@@ -673,10 +889,8 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
     delete h;
     return DWFGPU_HIP_ERROR;
   }
-  r = precision == DWFGPU_F64 ? upload_gauge<double>(h, gauge, bc) : upload_gauge<float>(h, gauge, bc);
-  if (r) return fail(r);
-  if ((r = alloc(h, &h->te, h->n * h->elem))) return fail(r);
-  if ((r = alloc(h, &h->t1, h->n * h->elem))) return fail(r);
+  if ((r = upload_gauge(h, gauge))) return fail(r);
+  if ((r = ensure_api(h, 0))) return fail(r);
   if ((r = alloc(h, &h->cps_stage, h->n * sizeof(double)))) return fail(r);
   if ((r = alloc(h, (void**)&h->partial, h->npartial * sizeof(double)))) return fail(r);
   if ((r = alloc(h, (void**)&h->dscal, 8 * sizeof(double)))) return fail(r);
@@ -705,26 +919,26 @@ size_t dwfgpu_half_size(dwfgpu_t h) { return h ? h->n : 0; }
 int dwfgpu_matpcdagmatpc(dwfgpu_t h, double mass, double* out, const double* in,
                          double* dot_prd) {
   if (!h || !out || !in) return DWFGPU_BAD_ARG;
-  TRY(ensure_work(h));
-  TRY(DISPATCH(h, to_device(h, h->work[0], in)));
-  TRY(DISPATCH(h, mdagm(h, mass, h->work[1], h->work[0], dot_prd)));
-  return DISPATCH(h, to_host(h, out, h->work[1]));
+  TRY(ensure_api(h, 2));
+  TRY(DISPATCH(h, to_device(h, api_work(h, 0), in)));
+  TRY(DISPATCH(h, mdagm(h, mass, api_work(h, 1), api_work(h, 0), dot_prd)));
+  return DISPATCH(h, to_host(h, out, api_work(h, 1)));
 }
 
 int dwfgpu_matpc(dwfgpu_t h, double mass, double* out, const double* in, int dag) {
   if (!h || !out || !in || (dag != 0 && dag != 1)) return DWFGPU_BAD_ARG;
-  TRY(ensure_work(h));
-  TRY(DISPATCH(h, to_device(h, h->work[0], in)));
-  TRY(DISPATCH(h, matpc(h, mass, h->work[1], h->work[0], dag, false)));
-  return DISPATCH(h, to_host(h, out, h->work[1]));
+  TRY(ensure_api(h, 2));
+  TRY(DISPATCH(h, to_device(h, api_work(h, 0), in)));
+  TRY(DISPATCH(h, matpc(h, mass, api_work(h, 1), api_work(h, 0), dag, false)));
+  return DISPATCH(h, to_host(h, out, api_work(h, 1)));
 }
 
 int dwfgpu_dslash(dwfgpu_t h, double mass, double* out, const double* in, int cb, int dag) {
   if (!h || !out || !in || (dag != 0 && dag != 1) || (cb != 0 && cb != 1)) return DWFGPU_BAD_ARG;
-  TRY(ensure_work(h));
-  TRY(DISPATCH(h, to_device(h, h->work[0], in)));
-  TRY(DISPATCH(h, dslash(h, mass, h->work[1], h->work[0], cb, dag)));
-  return DISPATCH(h, to_host(h, out, h->work[1]));
+  TRY(ensure_api(h, 2));
+  TRY(DISPATCH(h, to_device(h, api_work(h, 0), in)));
+  TRY(DISPATCH(h, dslash(h, mass, api_work(h, 1), api_work(h, 0), cb, dag)));
+  return DISPATCH(h, to_host(h, out, api_work(h, 1)));
 }
 
 int dwfgpu_invcg(dwfgpu_t h, const dwfgpu_cg_arg* arg, double* sol, const double* src,
@@ -734,9 +948,9 @@ int dwfgpu_invcg(dwfgpu_t h, const dwfgpu_cg_arg* arg, double* sol, const double
     set_err("dwfgpu_invcg: max_num_iter must be >= 1");
     return DWFGPU_BAD_ARG;
   }
-  TRY(ensure_work(h));
-  void* dsol = h->work[3];
-  void* dsrc = h->work[4];
+  TRY(ensure_api(h, 5));
+  const DVec& dsol = api_work(h, 3);
+  const DVec& dsrc = api_work(h, 4);
   TRY(DISPATCH(h, to_device(h, dsol, sol)));
   TRY(DISPATCH(h, to_device(h, dsrc, src)));
   const int st = DISPATCH(h, invcg(h, arg, dsol, dsrc, true_res, iters));
@@ -749,28 +963,44 @@ int dwfgpu_matinv(dwfgpu_t h, const dwfgpu_cg_arg* arg, double* out, const doubl
                   double* true_res, int* iters) {
   if (!h || !arg || !out || !in) return DWFGPU_BAD_ARG;
   if (arg->max_num_iter < 1) return DWFGPU_BAD_ARG;
-  TRY(ensure_work(h));
-  void *i1, *i2, *o1, *o2;
-  const size_t bytes = h->n * h->elem;
-  HIPCHK(hipMalloc(&i1, bytes));
-  HIPCHK(hipMalloc(&i2, bytes));
-  HIPCHK(hipMalloc(&o1, bytes));
-  HIPCHK(hipMalloc(&o2, bytes));
-  int st = DISPATCH(h, to_device(h, i1, in));
-  if (!st) st = DISPATCH(h, to_device(h, i2, in + h->n));
-  if (!st) st = DISPATCH(h, to_device(h, o1, out));
-  if (!st) st = DISPATCH(h, to_device(h, o2, out + h->n));
+  TRY(ensure_api(h, 6));
+  DVec v[4];
+  int st = DWFGPU_OK;
+  for (int k = 0; k < 4 && !st; ++k)
+    st = alloc_api_vec(h, &v[k]);
+  if (!st) st = DISPATCH(h, to_device(h, v[0], in));
+  if (!st) st = DISPATCH(h, to_device(h, v[1], in + h->n));
+  if (!st) st = DISPATCH(h, to_device(h, v[2], out));
+  if (!st) st = DISPATCH(h, to_device(h, v[3], out + h->n));
   // the CG works in work[0..2]; MatInv temporaries are work[3..5]
-  if (!st) st = DISPATCH(h, matinv(h, arg, o1, o2, i1, i2, true_res, iters));
+  if (!st) st = DISPATCH(h, matinv(h, arg, v[2], v[3], v[0], v[1], true_res, iters));
   if (st == DWFGPU_OK || st == DWFGPU_MAXITER) {
-    int r = DISPATCH(h, to_host(h, out, o1));
-    if (!r) r = DISPATCH(h, to_host(h, out + h->n, o2));
+    int r = DISPATCH(h, to_host(h, out, v[2]));
+    if (!r) r = DISPATCH(h, to_host(h, out + h->n, v[3]));
     if (r) st = r;
   }
-  (void)hipFree(i1);
-  (void)hipFree(i2);
-  (void)hipFree(o1);
-  (void)hipFree(o2);
+  (void)hipStreamSynchronize(h->st);
+  for (auto& x : v) free_vec(&x);
+  return st;
+}
+
+int dwfgpu_invcg_mixed(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, double delta,
+                       double* sol, const double* src, double* true_res, int* iters,
+                       int* reliable_updates) {
+  if (!h || !arg || !sol || !src) return DWFGPU_BAD_ARG;
+  dwfgpu_vec_t vs = nullptr, vb = nullptr;
+  TRY(dwfgpu_vec_create(h, &vs));
+  int st = dwfgpu_vec_create(h, &vb);
+  if (!st) st = dwfgpu_vec_upload(vs, sol);
+  if (!st) st = dwfgpu_vec_upload(vb, src);
+  if (!st) st = dwfgpu_invcg_mixed_dev(h, arg, sloppy, delta, vs, vb, true_res, iters,
+                                       reliable_updates);
+  if (st == DWFGPU_OK || st == DWFGPU_MAXITER) {
+    const int r = dwfgpu_vec_download(vs, sol);
+    if (r) st = r;
+  }
+  if (vs) dwfgpu_vec_destroy(vs);
+  if (vb) dwfgpu_vec_destroy(vb);
   return st;
 }
 
@@ -778,12 +1008,12 @@ int dwfgpu_vec_create(dwfgpu_t h, dwfgpu_vec_t* v) {
   if (!h || !v) return DWFGPU_BAD_ARG;
   dwfgpu_vec_s* p = new dwfgpu_vec_s;
   p->h = h;
-  if (hipMalloc(&p->d, h->n * h->elem) != hipSuccess) {
+  const int st = alloc_api_vec(h, &p->v);
+  if (st) {
+    free_vec(&p->v);
     delete p;
-    set_err("dwfgpu_vec_create: hipMalloc of %zu bytes failed", h->n * h->elem);
-    return DWFGPU_HIP_ERROR;
+    return st;
   }
-  HIPCHK(hipMemsetAsync(p->d, 0, h->n * h->elem, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   *v = p;
   return DWFGPU_OK;
@@ -792,27 +1022,27 @@ int dwfgpu_vec_create(dwfgpu_t h, dwfgpu_vec_t* v) {
 int dwfgpu_vec_destroy(dwfgpu_vec_t v) {
   if (!v) return DWFGPU_BAD_ARG;
   (void)hipStreamSynchronize(v->h->st);
-  (void)hipFree(v->d);
+  free_vec(&v->v);
   delete v;
   return DWFGPU_OK;
 }
 
 int dwfgpu_vec_upload(dwfgpu_vec_t v, const double* host) {
   if (!v || !host) return DWFGPU_BAD_ARG;
-  TRY(DISPATCH(v->h, to_device(v->h, v->d, host)));
+  TRY(DISPATCH(v->h, to_device(v->h, v->v, host)));
   HIPCHK(hipStreamSynchronize(v->h->st));
   return DWFGPU_OK;
 }
 
 int dwfgpu_vec_download(dwfgpu_vec_t v, double* host) {
   if (!v || !host) return DWFGPU_BAD_ARG;
-  return DISPATCH(v->h, to_host(v->h, host, v->d));
+  return DISPATCH(v->h, to_host(v->h, host, v->v));
 }
 
 int dwfgpu_matpcdagmatpc_dev(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_vec_t in,
                              double* dot_prd) {
   if (!h || !out || !in || out->h != h || in->h != h) return DWFGPU_BAD_ARG;
-  TRY(DISPATCH(h, mdagm(h, mass, out->d, in->d, dot_prd)));
+  TRY(DISPATCH(h, mdagm(h, mass, out->v, in->v, dot_prd)));
   HIPCHK(hipStreamSynchronize(h->st));
   return DWFGPU_OK;
 }
@@ -821,17 +1051,49 @@ int dwfgpu_invcg_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, dwfgpu_vec_t sol, dwf
                      double* true_res, int* iters) {
   if (!h || !arg || !sol || !src || sol->h != h || src->h != h) return DWFGPU_BAD_ARG;
   if (arg->max_num_iter < 1) return DWFGPU_BAD_ARG;
-  TRY(ensure_work(h));
-  return DISPATCH(h, invcg(h, arg, sol->d, src->d, true_res, iters));
+  return DISPATCH(h, invcg(h, arg, sol->v, src->v, true_res, iters));
 }
 
-int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, double* bytes, double* flops) {
+int dwfgpu_invcg_mixed_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, double delta,
+                           dwfgpu_vec_t sol, dwfgpu_vec_t src, double* true_res, int* iters,
+                           int* reliable_updates) {
+  if (!h || !arg || !sol || !src || sol->h != h || src->h != h) return DWFGPU_BAD_ARG;
+  if (arg->max_num_iter < 1 || !(delta > 0.0 && delta < 1.0)) {
+    set_err("dwfgpu_invcg_mixed: need max_num_iter >= 1 and 0 < delta < 1");
+    return DWFGPU_BAD_ARG;
+  }
+  if (h->prec != DWFGPU_F64) {
+    set_err("dwfgpu_invcg_mixed: the handle's API precision must be DWFGPU_F64");
+    return DWFGPU_BAD_ARG;
+  }
+  switch (sloppy) {
+    case DWFGPU_F64:
+      return Ops<P64>::invcg(h, arg, sol->v, src->v, true_res, iters);
+    case DWFGPU_F32:
+      return invcg_mixed<P32>(h, arg, delta, sol->v, src->v, true_res, iters, reliable_updates);
+    case DWFGPU_BF16:
+      return invcg_mixed<PBF16>(h, arg, delta, sol->v, src->v, true_res, iters, reliable_updates);
+    case DWFGPU_I16:
+      return invcg_mixed<PI16>(h, arg, delta, sol->v, src->v, true_res, iters, reliable_updates);
+    default:
+      set_err("dwfgpu_invcg_mixed: unknown sloppy precision %d", sloppy);
+      return DWFGPU_BAD_ARG;
+  }
+}
+
+int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, int precision, double* bytes, double* flops) {
   if (!h) return DWFGPU_BAD_ARG;
+  double half, links;
+  switch (precision) {
+    case DWFGPU_F64: half = half_bytes<P64>(h); links = link_bytes<P64>(h); break;
+    case DWFGPU_F32: half = half_bytes<P32>(h); links = link_bytes<P32>(h); break;
+    case DWFGPU_BF16: half = half_bytes<PBF16>(h); links = link_bytes<PBF16>(h); break;
+    case DWFGPU_I16: half = half_bytes<PI16>(h); links = link_bytes<PI16>(h); break;
+    default: return DWFGPU_BAD_ARG;
+  }
+  // 10 half-vector passes and 4 link passes (include/roofline.h)
+  if (bytes) *bytes = 10.0 * half + 4.0 * links;
   const double nh = (double)h->g.vh * h->ls;
-  const double v4 = 2.0 * h->g.vh;
-  if (bytes)
-    *bytes = DWF_MDAGM_SPINOR_REALS_PER_NH * nh * h->elem +
-             DWF_MDAGM_LINK_REALS_PER_V4 * v4 * h->elem;
   if (flops)
     *flops = nh * (h->action == DWFGPU_DWF ? DWF_MDAGM_FLOPS_PER_NH_DWF
                                            : DWF_MDAGM_FLOPS_PER_NH_MOBIUS);
@@ -874,14 +1136,47 @@ int dwfgpu_time_copy(dwfgpu_t h, size_t bytes, int reps, double* gbs) {
   return st;
 }
 
-int dwfgpu_time_matpcdagmatpc(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_vec_t in,
-                              int reps, double* ms_app, double* ms_kernel, double* bytes_kernel,
-                              int* nkernel) {
+int dwfgpu_time_matpcdagmatpc(dwfgpu_t h, int precision, double mass, dwfgpu_vec_t out,
+                              dwfgpu_vec_t in, int reps, double* ms_app, double* ms_kernel,
+                              double* bytes_kernel, int* nkernel) {
   if (!h || !out || !in || reps < 1 || out->h != h || in->h != h) return DWFGPU_BAD_ARG;
-  return h->prec == DWFGPU_F64 ? time_mdagm<double>(h, mass, out->d, in->d, reps, ms_app, ms_kernel,
-                                                    bytes_kernel, nkernel)
-                               : time_mdagm<float>(h, mass, out->d, in->d, reps, ms_app, ms_kernel,
-                                                   bytes_kernel, nkernel);
+  if (precision == h->prec) {
+    switch (h->prec) {
+      case DWFGPU_F64:
+        return time_mdagm<P64>(h, mass, out->v, in->v, reps, ms_app, ms_kernel, bytes_kernel,
+                               nkernel);
+      case DWFGPU_F32:
+        return time_mdagm<P32>(h, mass, out->v, in->v, reps, ms_app, ms_kernel, bytes_kernel,
+                               nkernel);
+      case DWFGPU_BF16:
+        return time_mdagm<PBF16>(h, mass, out->v, in->v, reps, ms_app, ms_kernel, bytes_kernel,
+                                 nkernel);
+      default:
+        return time_mdagm<PI16>(h, mass, out->v, in->v, reps, ms_app, ms_kernel, bytes_kernel,
+                                nkernel);
+    }
+  }
+  if (h->prec != DWFGPU_F64) {
+    set_err("dwfgpu_time_matpcdagmatpc: a precision other than the handle's needs an FP64 handle");
+    return DWFGPU_BAD_ARG;
+  }
+  // time a sloppy-precision copy of `in` (converted on the device)
+  auto run = [&](auto tag) -> int {
+    typedef decltype(tag) P;
+    TRY(ensure_work<P>(h, 2));
+    const DVec& a = h->f[pidx<P>()].work[0];
+    const DVec& b = h->f[pidx<P>()].work[1];
+    hipLaunchKernelGGL((dwfk::k_convert<P, P64>), blas_grid(), blas_block(), 0, h->st, fld<P>(a),
+                       cfld<P64>(in->v), h->g);
+    HIPCHK(hipGetLastError());
+    return time_mdagm<P>(h, mass, b, a, reps, ms_app, ms_kernel, bytes_kernel, nkernel);
+  };
+  switch (precision) {
+    case DWFGPU_F32: return run(P32());
+    case DWFGPU_BF16: return run(PBF16());
+    case DWFGPU_I16: return run(PI16());
+    default: return DWFGPU_BAD_ARG;
+  }
 }
 
 }  // extern "C"

@@ -1,19 +1,29 @@
 // dwf_kernels.hpp -- gfx950 device code for the even-odd preconditioned
-// domain-wall / Mobius operator.
+// domain-wall / Mobius operator and its CG vector algebra.
 //
-// Device layout (structure of arrays, one complex number per 8/16-byte word):
-//   half vector  v[((s*12 + c)*vh + i)]     c = spin*3 + colour, i = site index
-//   gauge        U[((mu*2 + parity)*9 + row*3 + col)*vh + i]   (canonical U(row,col),
-//                boundary signs already folded in)
+// Fields are stored in one of four policies (compute type R, stored type S):
+//   P64   : FP64 spinors, FP64 links (18 reals)
+//   P32   : FP32 spinors, FP32 links compressed to their first two rows
+//   PBF16 : bfloat16 spinors with one FP32 scale per (site, slice), bf16
+//           compressed links
+//   PI16  : int16 fixed-point spinors with one FP32 scale per (site, slice),
+//           int16 compressed links (|U_ij| <= 1)
+// Compressed links are rebuilt as row2 = conj(row0 x row1) times the
+// anti-periodic boundary sign when they are staged in LDS.
+//
+// Device layout (structure of arrays):
+//   spinor data   d[(s*12 + c)*vh + i]   c = spin*3 + colour, one complex per word
+//   spinor scale  sc[s*vh + i]           (scaled policies only)
+//   links         U[((mu*2 + parity)*NE + e)*vh + i]   NE = 9 (P64) or 6
+//                 e = row*3 + col of canonical U(row, col), boundary signs folded in
 // Site index of a checkerboard site (z- and t-tiled, fastest first):
 //   i = x/2 + lx/2*(t%bt + bt*(z%bz + bz*(y + ly*(z/bz + lz/bz*(t/bt)))))
-// so y-, z- and t-hops inside a bz x bt tile stay within a few hundred sites
-// and the neighbours of a work-group stay within one XCD's L2.
-// A thread owns one output site (i, s); a work-group holds NS consecutive
-// sites times all Ls slices, so the 8 links of a site are fetched once per
-// group and served to all slices from L1/L2, and a whole s-column of a site
-// sits in one group (needed by the Mobius M5^-1 recurrence, done in LDS).
-// Work-groups are renumbered so each XCD sweeps one contiguous range.
+// so y-, z- and t-hops inside a bz x bt tile stay within a few hundred sites.
+// A thread owns one output site (i, s); a work-group holds ns consecutive
+// sites times all Ls slices.  The group stages the 8 links of each of its
+// sites in LDS once (all slices share them), and the Mobius M5^-1 and
+// (b + c D5) epilogues run on the s-columns of its sites in LDS.
+// Work-groups are renumbered so each XCD sweeps one contiguous site range.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -52,10 +62,138 @@ __device__ __forceinline__ cx<R> sgr(cx<R> a) {
   return SG > 0 ? a : cx<R>{-a.re, -a.im};
 }
 
+// ---------------------------------------------------------------------------
+// storage policies
+// ---------------------------------------------------------------------------
+struct P64 {
+  typedef double R;
+  typedef cx<double> S;
+  static constexpr bool kScaled = false;
+  static constexpr int kLinkElems = 9;
+};
+struct P32 {
+  typedef float R;
+  typedef cx<float> S;
+  static constexpr bool kScaled = false;
+  static constexpr int kLinkElems = 6;
+};
+struct PBF16 {
+  typedef float R;
+  typedef uint32_t S;  // re in the low half, im in the high half
+  static constexpr bool kScaled = true;
+  static constexpr int kLinkElems = 6;
+};
+struct PI16 {
+  typedef float R;
+  typedef uint32_t S;
+  static constexpr bool kScaled = true;
+  static constexpr int kLinkElems = 6;
+};
+
+// scale stored per (site, slice) from the largest |component| m of the site
+template <class P>
+__device__ __forceinline__ float scale_of(float) { return 1.0f; }
+template <>
+__device__ __forceinline__ float scale_of<PBF16>(float m) { return m; }
+template <>
+__device__ __forceinline__ float scale_of<PI16>(float m) { return m * (1.0f / 32767.0f); }
+
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+  uint32_t b = __float_as_uint(x);
+  b += 0x7fffu + ((b >> 16) & 1u);
+  return b >> 16;
+}
+
+template <class P>
+struct Pack;
+template <>
+struct Pack<P64> {
+  __device__ static cx<double> unpack(cx<double> v, float) { return v; }
+  __device__ static cx<double> pack(cx<double> v, float) { return v; }
+};
+template <>
+struct Pack<P32> {
+  __device__ static cx<float> unpack(cx<float> v, float) { return v; }
+  __device__ static cx<float> pack(cx<float> v, float) { return v; }
+};
+template <>
+struct Pack<PBF16> {
+  __device__ static cx<float> unpack(uint32_t v, float sc) {
+    return {__uint_as_float(v << 16) * sc, __uint_as_float(v & 0xffff0000u) * sc};
+  }
+  __device__ static uint32_t pack(cx<float> v, float inv) {
+    return bf16_rne(v.re * inv) | (bf16_rne(v.im * inv) << 16);
+  }
+};
+template <>
+struct Pack<PI16> {
+  __device__ static cx<float> unpack(uint32_t v, float sc) {
+    return {(float)(int16_t)(v & 0xffffu) * sc, (float)(int16_t)(v >> 16) * sc};
+  }
+  __device__ static uint32_t pack(cx<float> v, float inv) {
+    const int a = max(-32767, min(32767, __float2int_rn(v.re * inv)));
+    const int b = max(-32767, min(32767, __float2int_rn(v.im * inv)));
+    return ((uint32_t)a & 0xffffu) | ((uint32_t)b << 16);
+  }
+};
+// fixed scale of compressed 16-bit links
+template <class P>
+__device__ __forceinline__ float link_scale() { return 1.0f; }
+template <>
+__device__ __forceinline__ float link_scale<PI16>() { return 1.0f / 32767.0f; }
+
+// a spinor field: data + per-(site,slice) scales (scaled policies only)
+template <class P>
+struct Fld {
+  typename P::S* d;
+  float* sc;
+};
+template <class P>
+struct CFld {
+  const typename P::S* d;
+  const float* sc;
+};
+
+// complex component c of site n in the slice starting at (d, sc)
+template <class P>
+__device__ __forceinline__ cx<typename P::R> ld(const typename P::S* __restrict__ d,
+                                                const float* __restrict__ sc, int vh, int c,
+                                                int n, float s) {
+  (void)sc;
+  return Pack<P>::unpack(d[(size_t)c * vh + n], s);
+}
+template <class P>
+__device__ __forceinline__ float ld_scale(const float* __restrict__ sc, int n) {
+  return P::kScaled ? sc[n] : 1.0f;
+}
+
+// store the 12 complex components of site n of a slice
+template <class P>
+__device__ __forceinline__ void st_site(typename P::S* __restrict__ d, float* __restrict__ sc,
+                                        int vh, int n, const cx<typename P::R>* v) {
+  if constexpr (P::kScaled) {
+    float m = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) m = fmaxf(m, fmaxf(fabsf((float)v[c].re), fabsf((float)v[c].im)));
+    const float s = scale_of<P>(m);
+    const float inv = m > 0.0f ? 1.0f / s : 0.0f;
+    sc[n] = s;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) d[(size_t)c * vh + n] = Pack<P>::pack(v[c], inv);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 12; ++c) d[(size_t)c * vh + n] = Pack<P>::pack(v[c], 1.0f);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
 struct Geom {
   int lx, ly, lz, lt, lxh, vh, ls;
   int bt, bz;  // time / z slices per tile of the site index (divide lt / lz)
   int nxcd;    // XCDs the work-groups are spread over
+  int bcmask;  // bit mu set: anti-periodic sign on the links leaving x_mu = L_mu - 1
 };
 
 // site index of (x/2, y, z, t): order (fastest first) x/2, t%bt, z%bz, y,
@@ -77,11 +215,8 @@ __host__ __device__ __forceinline__ int site_of_cps(const Geom& g, int c) {
   return site_index(g, xh, y, z, t);
 }
 
-struct Nbr {
-  int fwd[4], bwd[4];
-};
-
-__host__ __device__ __forceinline__ Nbr neighbours(const Geom& g, int i, int parity) {
+// coordinates of site index i of parity `parity`
+__host__ __device__ __forceinline__ void site_coords(const Geom& g, int i, int parity, int* c) {
   const int xh = i % g.lxh;
   int r = i / g.lxh;
   const int ti = r % g.bt;
@@ -93,7 +228,20 @@ __host__ __device__ __forceinline__ Nbr neighbours(const Geom& g, int i, int par
   const int nzb = g.lz / g.bz;
   const int z = (r % nzb) * g.bz + zi;
   const int t = (r / nzb) * g.bt + ti;
-  const int x = 2 * xh + ((parity + y + z + t) & 1);
+  c[0] = 2 * xh + ((parity + y + z + t) & 1);
+  c[1] = y;
+  c[2] = z;
+  c[3] = t;
+}
+
+struct Nbr {
+  int fwd[4], bwd[4];
+};
+
+__host__ __device__ __forceinline__ Nbr neighbours(const Geom& g, int i, int parity) {
+  int c[4];
+  site_coords(g, i, parity, c);
+  const int x = c[0], y = c[1], z = c[2], t = c[3], xh = x >> 1;
   const int xp = (x + 1 == g.lx) ? 0 : x + 1, xm = (x == 0) ? g.lx - 1 : x - 1;
   const int yp = (y + 1 == g.ly) ? 0 : y + 1, ym = (y == 0) ? g.ly - 1 : y - 1;
   const int zp = (z + 1 == g.lz) ? 0 : z + 1, zm = (z == 0) ? g.lz - 1 : z - 1;
@@ -121,6 +269,9 @@ __device__ __forceinline__ int xcd_block(int nxcd) {
   return k * q + (k < r ? k : r) + j;
 }
 
+// ---------------------------------------------------------------------------
+// Wilson hopping term
+// ---------------------------------------------------------------------------
 // (1 + SG gamma_mu) psi -> two-spinor h (spins 0,1), gamma matrices of the
 // chiral basis CPS uses (cps_pp/include/util/dirac_op.h:956-975).
 template <int MU, int SG, typename R>
@@ -166,42 +317,72 @@ __device__ __forceinline__ void reconstruct(cx<R>* a, const cx<R>* r) {
   }
 }
 
-// plain load of the input spinor at checkerboard site n of one slice
-template <typename R>
+// plain load of the input spinor at site n of one slice
+template <class P>
 struct PlainLoader {
-  const cx<R>* __restrict__ base;
+  const typename P::S* __restrict__ d;
+  const float* __restrict__ sc;
   int vh;
-  __device__ __forceinline__ void operator()(int n, cx<R>* p) const {
+  __device__ __forceinline__ void operator()(int n, cx<typename P::R>* p) const {
+    const float s = ld_scale<P>(sc, n);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) p[c] = base[c * vh + n];
+    for (int c = 0; c < 12; ++c) p[c] = ld<P>(d, sc, vh, c, n, s);
   }
 };
 
 // on-the-fly (b + c D5) of zmobius_dslash_4_dag0 (zmobius_dslash_4.C:90-105):
 // t(s) = b in(s) + c P+ in(s-1) + c P- in(s+1), walls carry -m c.
-template <typename R>
+template <class P>
 struct MobTLoader {
-  const cx<R>* __restrict__ cur;
-  const cx<R>* __restrict__ prv;
-  const cx<R>* __restrict__ nxt;
+  typedef typename P::R R;
+  const typename P::S* __restrict__ cur;
+  const typename P::S* __restrict__ prv;
+  const typename P::S* __restrict__ nxt;
+  const float* __restrict__ sc_cur;
+  const float* __restrict__ sc_prv;
+  const float* __restrict__ sc_nxt;
   int vh;
   R b, cm, cp;
   __device__ __forceinline__ void operator()(int n, cx<R>* p) const {
+    const float s0 = ld_scale<P>(sc_cur, n), sm = ld_scale<P>(sc_prv, n),
+                sp = ld_scale<P>(sc_nxt, n);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) p[c] = b * cur[c * vh + n] + cm * prv[c * vh + n];
+    for (int c = 0; c < 6; ++c)
+      p[c] = b * ld<P>(cur, sc_cur, vh, c, n, s0) + cm * ld<P>(prv, sc_prv, vh, c, n, sm);
 #pragma unroll
-    for (int c = 6; c < 12; ++c) p[c] = b * cur[c * vh + n] + cp * nxt[c * vh + n];
+    for (int c = 6; c < 12; ++c)
+      p[c] = b * ld<P>(cur, sc_cur, vh, c, n, s0) + cp * ld<P>(nxt, sc_nxt, vh, c, n, sp);
   }
 };
 
-// Links of the work-group's sites, staged once in LDS and shared by all Ls
-// slices: lu[((set*8 + d)*ns + site)*9 + e], d < 4 the forward link U_d(x),
-// d >= 4 the backward link U_{d-4}(x - d) of the output site x; `set`
-// distinguishes the two output parities of a DWF work-group.
-template <typename R>
-__device__ __forceinline__ void stage_links(cx<R>* __restrict__ lu, const cx<R>* __restrict__ U,
-                                            const Geom& g, int blk, int ns, int nsets,
-                                            int p_first) {
+// site index of the link U_MU(x) (bwd: U_MU(x - MU)) of the site with
+// coordinates c, and whether it carries the anti-periodic boundary sign
+template <int MU>
+__device__ __forceinline__ void link_site(const Geom& g, const int* c, bool bwd, int* site,
+                                          bool* neg) {
+  const int L = MU == 0 ? g.lx : MU == 1 ? g.ly : MU == 2 ? g.lz : g.lt;
+  const int x = c[MU];
+  const bool bc = (g.bcmask >> MU) & 1;
+  if (!bwd) {
+    *neg = bc && x == L - 1;
+    return;
+  }
+  *neg = bc && x == 0;
+  const int m = x == 0 ? L - 1 : x - 1;
+  *site = site_index(g, (MU == 0 ? m : c[0]) >> 1, MU == 1 ? m : c[1], MU == 2 ? m : c[2],
+                     MU == 3 ? m : c[3]);
+}
+
+// Links of the work-group's sites, staged once in LDS (compute type) and
+// shared by all Ls slices: lu[((set*8 + d)*ns + site)*9 + e], d < 4 the
+// forward link U_d(x), d >= 4 the backward link U_{d-4}(x - d) of the output
+// site x; `set` distinguishes the two output parities of a DWF work-group.
+template <class P>
+__device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
+                                            const typename P::S* __restrict__ U, const Geom& g,
+                                            int blk, int ns, int nsets, int p_first) {
+  typedef typename P::R R;
+  constexpr int NE = P::kLinkElems;
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
   const int nthr = blockDim.x * blockDim.y;
   for (int w = tid; w < nsets * 8 * ns; w += nthr) {
@@ -209,19 +390,36 @@ __device__ __forceinline__ void stage_links(cx<R>* __restrict__ lu, const cx<R>*
     const int i = blk * ns + il;
     if (i >= g.vh) continue;
     const int p = (p_first + set) & 1;
-    int site, par;
-    if (d < 4) {
-      site = i;
-      par = p;
-    } else {
-      site = neighbours(g, i, p).bwd[d - 4];
-      par = 1 - p;
-    }
     const int mu = d & 3;
-    const cx<R>* src = U + (size_t)((mu * 2 + par) * 9) * g.vh + site;
+    int c[4];
+    site_coords(g, i, p, c);
+    int site = i;
+    bool neg = false;
+    // one case per direction: a lane-dependent index into {lx, ly, lz, lt}
+    // would be lowered to a private (scratch) array
+    switch (mu) {
+      case 0: link_site<0>(g, c, d >= 4, &site, &neg); break;
+      case 1: link_site<1>(g, c, d >= 4, &site, &neg); break;
+      case 2: link_site<2>(g, c, d >= 4, &site, &neg); break;
+      default: link_site<3>(g, c, d >= 4, &site, &neg); break;
+    }
+    const int par = d >= 4 ? 1 - p : p;  // U_mu(x - mu) lives on the other parity
+    const typename P::S* src = U + (size_t)((mu * 2 + par) * NE) * g.vh + site;
     cx<R>* dst = lu + (size_t)w * 9;
+    const float ls = link_scale<P>();
 #pragma unroll
-    for (int e = 0; e < 9; ++e) dst[e] = src[(size_t)e * g.vh];
+    for (int e = 0; e < NE; ++e) dst[e] = Pack<P>::unpack(src[(size_t)e * g.vh], ls);
+    if (NE == 6) {
+      // row 2 = conj(row0 x row1) (det U = 1), times the boundary sign
+      const cx<R> a0 = dst[0], a1 = dst[1], a2 = dst[2], b0 = dst[3], b1 = dst[4], b2 = dst[5];
+      cx<R> r0 = cmul(a1, b2) - cmul(a2, b1);
+      cx<R> r1 = cmul(a2, b0) - cmul(a0, b2);
+      cx<R> r2 = cmul(a0, b1) - cmul(a1, b0);
+      const R sg = neg ? (R)-1 : (R)1;
+      dst[6] = cx<R>{sg * r0.re, -sg * r0.im};
+      dst[7] = cx<R>{sg * r1.re, -sg * r1.im};
+      dst[8] = cx<R>{sg * r2.re, -sg * r2.im};
+    }
   }
 }
 
@@ -288,16 +486,25 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;
 }
 
+template <typename R>
+__device__ __forceinline__ double norm12(const cx<R>* v) {
+  double n = 0.0;
+#pragma unroll
+  for (int c = 0; c < 12; ++c) n += (double)v[c].re * v[c].re + (double)v[c].im * v[c].im;
+  return n;
+}
+
 // ---------------------------------------------------------------------------
 // DWF: dwf_dslash = D4 on every slice with 4D parity (s+cb)%2 + 5th-dim hop
 // (dwf_dslash_4.C:107-137, dwf_dslash_5_plus_omp.C:115-270).
 // EPI 0: out = D in.   EPI 1: out = x0 + alpha D in.  NORM: |out|^2 partials.
 // ---------------------------------------------------------------------------
-template <typename R, int DAG, int EPI, bool NORM, int WPE>
-__global__ void __launch_bounds__(512, WPE)
-k_dwf_dslash(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __restrict__ U,
-             const cx<R>* __restrict__ x0, double* __restrict__ partial, Geom g, int cb_in,
-             double two_a5, double nm_two_a5, double alpha) {
+template <class P, int DAG, int EPI, bool NORM>
+__global__ void __launch_bounds__(512)
+k_dwf_dslash(Fld<P> out, CFld<P> in, const typename P::S* __restrict__ U, CFld<P> x0,
+             double* __restrict__ partial, Geom g, int cb_in, double two_a5, double nm_two_a5,
+             double alpha) {
+  typedef typename P::R R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[16];
   cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw);  // [2 parities][8][ns][9]
@@ -305,7 +512,7 @@ k_dwf_dslash(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>*
   const int blk = xcd_block(g.nxcd);
   const int i = blk * ns + il;
   // slice s holds output parity (s + 1 + cb_in) % 2: set s % 2 of the stage
-  stage_links(lu, U, g, blk, ns, 2, (1 + cb_in) & 1);
+  stage_links<P>(lu, U, g, blk, ns, 2, (1 + cb_in) & 1);
   __syncthreads();
   double nrm = 0.0;
   if (i < g.vh) {
@@ -314,34 +521,30 @@ k_dwf_dslash(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>*
     cx<R> acc[12];
 #pragma unroll
     for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
-    hop4<DAG>(acc, PlainLoader<R>{in + s * slice, g.vh}, lu + ((s & 1) * 8 * ns + il) * 9, ns * 9,
-              g, i, p_out);
-    // fifth dimension: the same checkerboard index i in slices s-1 / s+1
-    const cx<R>* sm = in + (s > 0 ? (s - 1) : (g.ls - 1)) * slice + i;
-    const cx<R>* sp = in + (s < g.ls - 1 ? (s + 1) : 0) * slice + i;
+    hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
+              lu + ((s & 1) * 8 * ns + il) * 9, ns * 9, g, i, p_out);
+    // fifth dimension: the same site index i in slices s-1 / s+1
+    const int sm = s > 0 ? s - 1 : g.ls - 1, spl = s < g.ls - 1 ? s + 1 : 0;
     const R cm = (R)(s > 0 ? two_a5 : nm_two_a5);
     const R cp = (R)(s < g.ls - 1 ? two_a5 : nm_two_a5);
+    const float scm = ld_scale<P>(in.sc + (size_t)sm * g.vh, i);
+    const float scp = ld_scale<P>(in.sc + (size_t)spl * g.vh, i);
     const int lo_m = DAG ? 6 : 0;  // components fed from s-1
     const int lo_p = DAG ? 0 : 6;  // components fed from s+1
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      acc[lo_m + c] = acc[lo_m + c] + cm * sm[(lo_m + c) * g.vh];
-      acc[lo_p + c] = acc[lo_p + c] + cp * sp[(lo_p + c) * g.vh];
+      acc[lo_m + c] = acc[lo_m + c] + cm * ld<P>(in.d + sm * slice, nullptr, g.vh, lo_m + c, i, scm);
+      acc[lo_p + c] = acc[lo_p + c] + cp * ld<P>(in.d + spl * slice, nullptr, g.vh, lo_p + c, i, scp);
     }
-    cx<R>* o = out + s * slice + i;
-    if (EPI == 0) {
-#pragma unroll
-      for (int c = 0; c < 12; ++c) o[c * g.vh] = acc[c];
-    } else {
-      const cx<R>* xs = x0 + s * slice + i;
+    if (EPI == 1) {
+      const float sx = ld_scale<P>(x0.sc + (size_t)s * g.vh, i);
       const R a = (R)alpha;
 #pragma unroll
-      for (int c = 0; c < 12; ++c) {
-        const cx<R> v = xs[c * g.vh] + a * acc[c];
-        o[c * g.vh] = v;
-        if (NORM) nrm += (double)v.re * v.re + (double)v.im * v.im;
-      }
+      for (int c = 0; c < 12; ++c)
+        acc[c] = ld<P>(x0.d + s * slice, nullptr, g.vh, c, i, sx) + a * acc[c];
+      if (NORM) nrm = norm12(acc);
     }
+    st_site<P>(out.d + s * slice, out.sc + (size_t)s * g.vh, g.vh, i, acc);
   }
   if (NORM) {
     const double t = block_sum(nrm, sh);
@@ -373,8 +576,8 @@ struct M5Side {
 };
 
 struct MobArgs {
-  double b, c, mc;     // Mobius b, c and -mass*c (wall coefficient)
-  double K, mK;        // kappa_b/kappa_c and -mass*K
+  double b, c, mc;  // Mobius b, c and -mass*c (wall coefficient)
+  double K, mK;     // kappa_b/kappa_c and -mass*K
   double kappa_b;
   M5Side m5;
 };
@@ -438,10 +641,32 @@ __device__ __forceinline__ void apply_t_col(cx<R>* tv, const cx<R>* col_site, in
   }
 }
 
-template <typename R, int DAG, int EPI, int FL, int WPE>
-__global__ void __launch_bounds__(512, WPE)
-k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __restrict__ psi,
-         const cx<R>* __restrict__ U, double* __restrict__ partial, Geom g, int p_out, MobArgs a) {
+// psi(s) + K D5^(dag) psi (s) at site i (the Schur "M5" term, zmobius_m-orig.h:96-121)
+template <class P>
+__device__ __forceinline__ void m5_term(cx<typename P::R>* v, CFld<P> psi, const Geom& g, int s,
+                                        int i, const MobArgs& a, int dag) {
+  typedef typename P::R R;
+  const size_t slice = (size_t)12 * g.vh;
+  const int ls = g.ls;
+  const int sm = s > 0 ? s - 1 : ls - 1, sp = s < ls - 1 ? s + 1 : 0;
+  const float s0 = ld_scale<P>(psi.sc + (size_t)s * g.vh, i);
+  const float fm = ld_scale<P>(psi.sc + (size_t)sm * g.vh, i);
+  const float fp = ld_scale<P>(psi.sc + (size_t)sp * g.vh, i);
+  const R km = (R)(s > 0 ? a.K : a.mK), kp = (R)(s < ls - 1 ? a.K : a.mK);
+#pragma unroll
+  for (int c = 0; c < 12; ++c) {
+    const bool from_minus = (c < 6) == (dag == 0);
+    const cx<R> d5 = from_minus ? km * ld<P>(psi.d + sm * slice, nullptr, g.vh, c, i, fm)
+                                : kp * ld<P>(psi.d + sp * slice, nullptr, g.vh, c, i, fp);
+    v[c] = ld<P>(psi.d + s * slice, nullptr, g.vh, c, i, s0) + d5;
+  }
+}
+
+template <class P, int DAG, int EPI, int FL>
+__global__ void __launch_bounds__(512)
+k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ U,
+         double* __restrict__ partial, Geom g, int p_out, MobArgs a) {
+  typedef typename P::R R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[16];
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
@@ -454,51 +679,50 @@ k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __r
   const int nvalid = min(ns, g.vh - blk * ns);
   const size_t slice = (size_t)12 * g.vh;
   const int ls = g.ls;
+  typename P::S* od = out.d + s * slice;
+  float* osc = out.sc + (size_t)s * g.vh;
 
-  stage_links(lu, U, g, blk, ns, 1, p_out);
+  stage_links<P>(lu, U, g, blk, ns, 1, p_out);
   __syncthreads();
   cx<R> acc[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
   if (act) {
     if (DAG == 0 && !(FL & FL_PRE_T)) {
-      MobTLoader<R> ld;
-      ld.cur = in + s * slice;
-      ld.prv = in + (s > 0 ? s - 1 : ls - 1) * slice;
-      ld.nxt = in + (s < ls - 1 ? s + 1 : 0) * slice;
+      MobTLoader<P> ld;
+      const int sm = s > 0 ? s - 1 : ls - 1, sp = s < ls - 1 ? s + 1 : 0;
+      ld.cur = in.d + s * slice;
+      ld.prv = in.d + sm * slice;
+      ld.nxt = in.d + sp * slice;
+      ld.sc_cur = in.sc + (size_t)s * g.vh;
+      ld.sc_prv = in.sc + (size_t)sm * g.vh;
+      ld.sc_nxt = in.sc + (size_t)sp * g.vh;
       ld.vh = g.vh;
       ld.b = (R)a.b;
       ld.cm = (R)(s > 0 ? a.c : a.mc);
       ld.cp = (R)(s < ls - 1 ? a.c : a.mc);
       hop4<0>(acc, ld, lu + il * 9, ns * 9, g, i, p_out);
     } else {
-      hop4<DAG>(acc, PlainLoader<R>{in + s * slice, g.vh}, lu + il * 9, ns * 9, g, i, p_out);
+      hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
+                lu + il * 9, ns * 9, g, i, p_out);
     }
   }
-  cx<R>* o = out + s * slice + i;
 
   if (DAG == 0 && EPI == EPI_PLAIN) {
-    if (act)
-#pragma unroll
-      for (int c = 0; c < 12; ++c) o[c * g.vh] = acc[c];
+    if (act) st_site<P>(od, osc, g.vh, i, acc);
     return;
   }
 
   if (DAG == 0 && EPI == EPI_SCHUR) {
     double nrm = 0.0;
     if (act) {
-      const cx<R>* p0 = psi + s * slice + i;
-      const cx<R>* pm = psi + (s > 0 ? s - 1 : ls - 1) * slice + i;
-      const cx<R>* pp = psi + (s < ls - 1 ? s + 1 : 0) * slice + i;
-      const R km = (R)(s > 0 ? a.K : a.mK), kp = (R)(s < ls - 1 ? a.K : a.mK);
+      cx<R> v[12];
+      m5_term<P>(v, psi, g, s, i, a, 0);
       const R nkb = (R)(-a.kappa_b);
 #pragma unroll
-      for (int c = 0; c < 12; ++c) {
-        const cx<R> d5 = (c < 6) ? km * pm[c * g.vh] : kp * pp[c * g.vh];
-        const cx<R> v = p0[c * g.vh] + d5 + nkb * acc[c];
-        o[c * g.vh] = v;
-        if (FL & FL_NORM) nrm += (double)v.re * v.re + (double)v.im * v.im;
-      }
+      for (int c = 0; c < 12; ++c) v[c] = v[c] + nkb * acc[c];
+      if (FL & FL_NORM) nrm = norm12(v);
+      st_site<P>(od, osc, g.vh, i, v);
     }
     if (FL & FL_NORM) {
       const double t = block_sum(nrm, sh);
@@ -516,22 +740,16 @@ k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __r
     cx<R> tv[12];
     apply_t_col(tv, mycol, s, ls, a, 1);
     if (EPI == EPI_PLAIN) {
-      if (act)
-#pragma unroll
-        for (int c = 0; c < 12; ++c) o[c * g.vh] = tv[c];
+      if (act) st_site<P>(od, osc, g.vh, i, tv);
       return;
     }
     if (EPI == EPI_SCHUR) {
       if (act) {
-        const cx<R>* p0 = psi + s * slice + i;
-        const cx<R>* pm = psi + (s > 0 ? s - 1 : ls - 1) * slice + i;
-        const cx<R>* pp = psi + (s < ls - 1 ? s + 1 : 0) * slice + i;
-        const R km = (R)(s > 0 ? a.K : a.mK), kp = (R)(s < ls - 1 ? a.K : a.mK);
+        cx<R> v[12];
+        m5_term<P>(v, psi, g, s, i, a, 1);
 #pragma unroll
-        for (int c = 0; c < 12; ++c) {
-          const cx<R> d5 = (c < 6) ? kp * pp[c * g.vh] : km * pm[c * g.vh];
-          o[c * g.vh] = tv[c] + p0[c * g.vh] + d5;
-        }
+        for (int c = 0; c < 12; ++c) v[c] = v[c] + tv[c];
+        st_site<P>(od, osc, g.vh, i, v);
       }
       return;
     }
@@ -548,24 +766,24 @@ k_mobius(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, const cx<R>* __r
   m5inv_columns<DAG>(col, sst, ns, nvalid, ls, a.m5);
   __syncthreads();
   if (act) {
+    cx<R> v[12];
     if (DAG == 0 && (FL & FL_POST_T)) {
-      cx<R> tv[12];
-      apply_t_col(tv, mycol, s, ls, a, 0);
-#pragma unroll
-      for (int c = 0; c < 12; ++c) o[c * g.vh] = tv[c];
+      apply_t_col(v, mycol, s, ls, a, 0);
     } else {
       const R sc = (R)(DAG ? a.kappa_b : 1.0);
 #pragma unroll
-      for (int c = 0; c < 12; ++c) o[c * g.vh] = sc * mycol[s * 12 + c];
+      for (int c = 0; c < 12; ++c) v[c] = sc * mycol[s * 12 + c];
     }
+    st_site<P>(od, osc, g.vh, i, v);
   }
 }
 
 // stand-alone M5^-1 (dag) on a half vector: zmobius_m5inv(out, in, ...)
 // (zmobius_m5inv.C:822-841)
-template <typename R, int DAG>
+template <class P, int DAG>
 __global__ void __launch_bounds__(512)
-k_m5inv(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, Geom g, M5Side k) {
+k_m5inv(Fld<P> out, CFld<P> in, Geom g, M5Side k) {
+  typedef typename P::R R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<R>* col = reinterpret_cast<cx<R>*>(smem_raw);
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
@@ -576,78 +794,121 @@ k_m5inv(cx<R>* __restrict__ out, const cx<R>* __restrict__ in, Geom g, M5Side k)
   const int nvalid = min(ns, g.vh - blk * ns);
   const size_t slice = (size_t)12 * g.vh;
   cx<R>* mycol = col + il * sst;
-  if (act)
+  if (act) {
+    const float sc = ld_scale<P>(in.sc + (size_t)s * g.vh, i);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = in[s * slice + c * g.vh + i];
+    for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = ld<P>(in.d + s * slice, nullptr, g.vh, c, i, sc);
+  }
   __syncthreads();
   m5inv_columns<DAG>(col, sst, ns, nvalid, g.ls, k);
   __syncthreads();
-  if (act)
-#pragma unroll
-    for (int c = 0; c < 12; ++c) out[s * slice + c * g.vh + i] = mycol[s * 12 + c];
+  if (act) st_site<P>(out.d + s * slice, out.sc + (size_t)s * g.vh, g.vh, i, mycol + s * 12);
 }
 
 // ---------------------------------------------------------------------------
-// BLAS-1 on flat arrays of n reals (vector_util.C:727-900), FP64 reductions
-// written as per-block partial sums (fixed grid => reproducible).
+// Vector algebra (vector_util.C:727-900) on whole half vectors, one thread per
+// (slice, site) with a grid-stride loop; FP64 reductions as per-block
+// partial sums over a fixed grid, so totals are reproducible.
 // ---------------------------------------------------------------------------
 constexpr int BLAS_BLOCKS = 1024;
 constexpr int BLAS_THREADS = 256;
 
-template <typename R>
+template <class P>
+__device__ __forceinline__ void ld_site(cx<typename P::R>* v, CFld<P> f, const Geom& g, size_t k) {
+  // k = s*vh + i
+  const size_t s = k / g.vh, i = k - s * g.vh;
+  const float sc = ld_scale<P>(f.sc, (int)k);
+  const typename P::S* d = f.d + s * 12 * (size_t)g.vh + i;
+#pragma unroll
+  for (int c = 0; c < 12; ++c) v[c] = Pack<P>::unpack(d[(size_t)c * g.vh], sc);
+}
+
+template <class P>
+__device__ __forceinline__ void st_site_k(Fld<P> f, const Geom& g, size_t k,
+                                          const cx<typename P::R>* v) {
+  const size_t s = k / g.vh, i = k - s * g.vh;
+  st_site<P>(f.d + s * 12 * (size_t)g.vh, f.sc + s * (size_t)g.vh, g.vh, (int)i, v);
+}
+
+#define DWFK_SITE_LOOP(k, g) \
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < (size_t)(g).vh * (g).ls; \
+       k += (size_t)gridDim.x * blockDim.x)
+
+template <class P>
 __global__ void __launch_bounds__(BLAS_THREADS)
-k_norm(const R* __restrict__ x, size_t n, double* __restrict__ partial) {
+k_norm(CFld<P> x, Geom g, double* __restrict__ partial) {
   __shared__ double sh[16];
   double acc = 0.0;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x)
-    acc += (double)x[k] * x[k];
+  DWFK_SITE_LOOP(k, g) {
+    cx<typename P::R> v[12];
+    ld_site<P>(v, x, g, k);
+    acc += norm12(v);
+  }
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
 // sol += a dir ; res -= a mmp ; partial |res|^2   (inv_cg.C:340-349)
-template <typename R>
+template <class P, class PX>
 __global__ void __launch_bounds__(BLAS_THREADS)
-k_cg_update(R* __restrict__ sol, R* __restrict__ res, const R* __restrict__ dir,
-            const R* __restrict__ mmp, double a, size_t n, double* __restrict__ partial) {
+k_cg_update(Fld<PX> sol, Fld<P> res, CFld<P> dir, CFld<P> mmp, double a, Geom g,
+            double* __restrict__ partial) {
+  typedef typename P::R R;
+  typedef typename PX::R RX;
   __shared__ double sh[16];
   double acc = 0.0;
-  const R ar = (R)a, mar = (R)(-a);
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x) {
-    sol[k] = ar * dir[k] + sol[k];
-    const R r = mar * mmp[k] + res[k];
-    res[k] = r;
-    acc += (double)r * r;
+  DWFK_SITE_LOOP(k, g) {
+    cx<R> d[12], m[12], r[12];
+    cx<RX> x[12];
+    ld_site<P>(d, dir, g, k);
+    ld_site<P>(m, mmp, g, k);
+    ld_site<P>(r, CFld<P>{res.d, res.sc}, g, k);
+    ld_site<PX>(x, CFld<PX>{sol.d, sol.sc}, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      x[c] = cx<RX>{(RX)(a * d[c].re) + x[c].re, (RX)(a * d[c].im) + x[c].im};
+      r[c] = cx<R>{(R)(-a * m[c].re) + r[c].re, (R)(-a * m[c].im) + r[c].im};
+    }
+    st_site_k<PX>(sol, g, k, x);
+    st_site_k<P>(res, g, k, r);
+    acc += norm12(r);
   }
   const double t = block_sum(acc, sh);
   if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
 // y = a x + b y   (fTimesV1PlusV2 and friends)
-template <typename R>
+template <class P>
 __global__ void __launch_bounds__(BLAS_THREADS)
-k_axpby(R* __restrict__ y, const R* __restrict__ x, double a, double b, size_t n) {
+k_axpby(Fld<P> y, CFld<P> x, double a, double b, Geom g) {
+  typedef typename P::R R;
   const R ar = (R)a, br = (R)b;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x)
-    y[k] = ar * x[k] + br * y[k];
+  DWFK_SITE_LOOP(k, g) {
+    cx<R> xv[12], yv[12];
+    ld_site<P>(xv, x, g, k);
+    ld_site<P>(yv, CFld<P>{y.d, y.sc}, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) yv[c] = ar * xv[c] + br * yv[c];
+    st_site_k<P>(y, g, k, yv);
+  }
 }
 
 // z = a x + y, partial |z|^2 optional
-template <typename R, bool NORM>
+template <class P, bool NORM>
 __global__ void __launch_bounds__(BLAS_THREADS)
-k_axpy_to(R* __restrict__ z, const R* __restrict__ x, const R* __restrict__ y, double a,
-          size_t n, double* __restrict__ partial) {
+k_axpy_to(Fld<P> z, CFld<P> x, CFld<P> y, double a, Geom g, double* __restrict__ partial) {
+  typedef typename P::R R;
   __shared__ double sh[16];
   double acc = 0.0;
   const R ar = (R)a;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x) {
-    const R v = ar * x[k] + y[k];
-    z[k] = v;
-    if (NORM) acc += (double)v * v;
+  DWFK_SITE_LOOP(k, g) {
+    cx<R> xv[12], yv[12];
+    ld_site<P>(xv, x, g, k);
+    ld_site<P>(yv, y, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) yv[c] = ar * xv[c] + yv[c];
+    st_site_k<P>(z, g, k, yv);
+    if (NORM) acc += norm12(yv);
   }
   if (NORM) {
     const double t = block_sum(acc, sh);
@@ -655,13 +916,49 @@ k_axpy_to(R* __restrict__ z, const R* __restrict__ x, const R* __restrict__ y, d
   }
 }
 
-// per-slice complex-free scale of a half vector by a real factor
-template <typename R>
-__global__ void __launch_bounds__(BLAS_THREADS) k_scale(R* __restrict__ x, double a, size_t n) {
+template <class P>
+__global__ void __launch_bounds__(BLAS_THREADS) k_scale(Fld<P> x, double a, Geom g) {
+  typedef typename P::R R;
   const R ar = (R)a;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
-       k += (size_t)gridDim.x * blockDim.x)
-    x[k] = ar * x[k];
+  DWFK_SITE_LOOP(k, g) {
+    cx<R> v[12];
+    ld_site<P>(v, CFld<P>{x.d, x.sc}, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) v[c] = ar * v[c];
+    st_site_k<P>(x, g, k, v);
+  }
+}
+
+// out = in in another storage precision
+template <class PO, class PI>
+__global__ void __launch_bounds__(BLAS_THREADS) k_convert(Fld<PO> out, CFld<PI> in, Geom g) {
+  typedef typename PO::R RO;
+  DWFK_SITE_LOOP(k, g) {
+    cx<typename PI::R> v[12];
+    cx<RO> w[12];
+    ld_site<PI>(v, in, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) w[c] = cx<RO>{(RO)v[c].re, (RO)v[c].im};
+    st_site_k<PO>(out, g, k, w);
+  }
+}
+
+// x (FP64) += xs (any precision); xs = 0 afterwards
+template <class PS>
+__global__ void __launch_bounds__(BLAS_THREADS) k_accumulate(Fld<P64> x, Fld<PS> xs, Geom g) {
+  DWFK_SITE_LOOP(k, g) {
+    cx<typename PS::R> v[12], z[12];
+    cx<double> w[12];
+    ld_site<PS>(v, CFld<PS>{xs.d, xs.sc}, g, k);
+    ld_site<P64>(w, CFld<P64>{x.d, x.sc}, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      w[c] = cx<double>{w[c].re + (double)v[c].re, w[c].im + (double)v[c].im};
+      z[c] = cx<typename PS::R>{0, 0};
+    }
+    st_site_k<P64>(x, g, k, w);
+    st_site_k<PS>(xs, g, k, z);
+  }
 }
 
 // reference streaming copy (16-byte words)
@@ -690,34 +987,55 @@ k_reduce_partials(const double* __restrict__ partial, int n, double* __restrict_
   if (threadIdx.x == 0) out[0] = t;
 }
 
-// CPS half vector [s][cb site][24 reals] (FP64, host order) <-> device SoA
-template <typename R>
+// CPS half vector [s][cb site][24 reals] (FP64, host order) <-> device field
+template <class P>
 __global__ void __launch_bounds__(256)
-k_cps_to_soa(cx<R>* __restrict__ soa, const double* __restrict__ cps, Geom g) {
-  const int vh = g.vh, ls = g.ls;
-  const size_t nsite = (size_t)vh * ls;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < nsite * 12;
-       k += (size_t)gridDim.x * blockDim.x) {
-    const size_t site = k / 12;  // s*vh + i
-    const int c = (int)(k % 12);
-    const size_t s = site / vh, i = site_of_cps(g, (int)(site % vh));
-    soa[(s * 12 + c) * vh + i] = cx<R>{(R)cps[k * 2], (R)cps[k * 2 + 1]};
+k_cps_to_field(Fld<P> f, const double* __restrict__ cps, Geom g) {
+  typedef typename P::R R;
+  DWFK_SITE_LOOP(k, g) {
+    const size_t s = k / g.vh;
+    const int ic = (int)(k - s * g.vh);
+    const double* src = cps + k * 24;
+    cx<R> v[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) v[c] = cx<R>{(R)src[2 * c], (R)src[2 * c + 1]};
+    st_site_k<P>(f, g, s * g.vh + site_of_cps(g, ic), v);
   }
 }
 
-template <typename R>
+template <class P>
 __global__ void __launch_bounds__(256)
-k_soa_to_cps(double* __restrict__ cps, const cx<R>* __restrict__ soa, Geom g) {
-  const int vh = g.vh, ls = g.ls;
-  const size_t nsite = (size_t)vh * ls;
-  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < nsite * 12;
+k_field_to_cps(double* __restrict__ cps, CFld<P> f, Geom g) {
+  typedef typename P::R R;
+  DWFK_SITE_LOOP(k, g) {
+    const size_t s = k / g.vh;
+    const int ic = (int)(k - s * g.vh);
+    cx<R> v[12];
+    ld_site<P>(v, f, g, s * g.vh + site_of_cps(g, ic));
+    double* dst = cps + k * 24;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      dst[2 * c] = (double)v[c].re;
+      dst[2 * c + 1] = (double)v[c].im;
+    }
+  }
+}
+
+// compressed links of policy P from the FP64 device links (rows 0 and 1)
+template <class P>
+__global__ void __launch_bounds__(256)
+k_compress_links(typename P::S* __restrict__ dst, const cx<double>* __restrict__ src, Geom g) {
+  const size_t n = (size_t)8 * g.vh;  // (mu, parity, site)
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x) {
-    const size_t site = k / 12;
-    const int c = (int)(k % 12);
-    const size_t s = site / vh, i = site_of_cps(g, (int)(site % vh));
-    const cx<R> v = soa[(s * 12 + c) * vh + i];
-    cps[k * 2] = (double)v.re;
-    cps[k * 2 + 1] = (double)v.im;
+    const size_t mp = k / g.vh, i = k - mp * g.vh;
+    const float inv = 1.0f / link_scale<P>();
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+      const cx<double> u = src[(mp * 9 + e) * g.vh + i];
+      dst[(mp * 6 + e) * g.vh + i] = Pack<P>::pack(cx<typename P::R>{(typename P::R)u.re,
+                                                                      (typename P::R)u.im}, inv);
+    }
   }
 }
 

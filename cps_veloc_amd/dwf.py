@@ -7,6 +7,7 @@ call surface of CPS (cps_pp/include/util/dirac_op.h):
     op.Dslash(out, inp, mass, cb, dag)
     iters, true_res = op.InvCg(out, inp, CgArg(mass, ...))   # dirac_op.h:138
     iters, true_res = op.MatInv(out, inp, CgArg(...))
+    iters, true_res, st, nrel = op.InvCgMixed_dev(sol, src, CgArg(...), "bf16")
 
 Vectors are float64 numpy arrays in CPS half-vector order.  The HIP library
 must be built (cps_veloc_amd/build.py); there is no CPU fallback.
@@ -24,8 +25,8 @@ LIB_PATH = os.path.join(_HERE, "libcps_dwf_gpu.so")
 
 OK, BREAKDOWN, MAXITER, BAD_ARG, HIP_ERROR = 0, 1, 2, 3, 4
 DWF, MOBIUS = 0, 1
-F64, F32 = 0, 1
-_PREC = {"f64": F64, "f32": F32}
+F64, F32, BF16, I16 = 0, 1, 2, 3
+_PREC = {"f64": F64, "f32": F32, "bf16": BF16, "i16": I16}
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _lib = None
@@ -36,6 +37,7 @@ EXPORTS = (
     "dwfgpu_matpcdagmatpc", "dwfgpu_matpc", "dwfgpu_dslash", "dwfgpu_invcg",
     "dwfgpu_matinv", "dwfgpu_vec_create", "dwfgpu_vec_destroy", "dwfgpu_vec_upload",
     "dwfgpu_vec_download", "dwfgpu_matpcdagmatpc_dev", "dwfgpu_invcg_dev",
+    "dwfgpu_invcg_mixed", "dwfgpu_invcg_mixed_dev",
     "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
 )
 # every entry point of include/dwf_synth.h
@@ -96,10 +98,16 @@ def lib():
         L.dwfgpu_matpcdagmatpc_dev.argtypes = [H, C.c_double, H, H, C.POINTER(C.c_double)]
         L.dwfgpu_invcg_dev.argtypes = [H, C.POINTER(CgArgC), H, H, C.POINTER(C.c_double),
                                        C.POINTER(C.c_int)]
-        L.dwfgpu_time_matpcdagmatpc.argtypes = [H, C.c_double, H, H, C.c_int,
+        L.dwfgpu_invcg_mixed.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, _dp, _dp,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int)]
+        L.dwfgpu_invcg_mixed_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, H, H,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int)]
+        L.dwfgpu_time_matpcdagmatpc.argtypes = [H, C.c_int, C.c_double, H, H, C.c_int,
                                                 C.POINTER(C.c_double), _dp, _dp,
                                                 C.POINTER(C.c_int)]
-        L.dwfgpu_matpcdagmatpc_cost.argtypes = [H, C.POINTER(C.c_double),
+        L.dwfgpu_matpcdagmatpc_cost.argtypes = [H, C.c_int, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double)]
         L.dwfgpu_time_copy.argtypes = [H, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.dwfsynth_gauge.argtypes = [_dp, C.c_int * 4, C.c_double, C.c_ulonglong]
@@ -159,6 +167,7 @@ class _DiracOpGpu:
         g = g.view(np.float64).reshape(-1) if np.iscomplexobj(g) else g.reshape(-1)
         self.dims = tuple(int(d) for d in dims)
         self.ls = int(ls)
+        self.precision = precision
         self.h = C.c_void_p()
         _chk(lib().dwfgpu_create(_arr(g), (C.c_int * 4)(*self.dims), self.ls, self.action,
                                  m5, a5inv, b, c, (C.c_int * 4)(*bc), _PREC[precision],
@@ -224,12 +233,36 @@ class _DiracOpGpu:
                                          C.byref(it)), "InvCg_dev", allow=(OK, MAXITER))
         return it.value, tr.value, st
 
-    def time_MatPcDagMatPc(self, out: DeviceVector, inp: DeviceVector, mass, reps):
+    def InvCgMixed(self, out, inp, cg_arg: CgArg, sloppy="f32", delta=0.1):
+        """FP64 reliable-update CG with a `sloppy` ("f64", "f32", "bf16", "i16")
+        operator.  out: initial guess in, solution out.
+        Returns (iters, true_res, status, reliable_updates)."""
+        tr, it, nr = C.c_double(), C.c_int(), C.c_int()
+        a = cg_arg.c()
+        st = _chk(lib().dwfgpu_invcg_mixed(self.h, C.byref(a), _PREC[sloppy], delta, out,
+                                           _arr(inp), C.byref(tr), C.byref(it), C.byref(nr)),
+                  "InvCgMixed", allow=(OK, MAXITER))
+        return it.value, tr.value, st, nr.value
+
+    def InvCgMixed_dev(self, sol: DeviceVector, src: DeviceVector, cg_arg: CgArg,
+                       sloppy="f32", delta=0.1):
+        tr, it, nr = C.c_double(), C.c_int(), C.c_int()
+        a = cg_arg.c()
+        st = _chk(lib().dwfgpu_invcg_mixed_dev(self.h, C.byref(a), _PREC[sloppy], delta, sol.h,
+                                               src.h, C.byref(tr), C.byref(it), C.byref(nr)),
+                  "InvCgMixed_dev", allow=(OK, MAXITER))
+        return it.value, tr.value, st, nr.value
+
+    def time_MatPcDagMatPc(self, out: DeviceVector, inp: DeviceVector, mass, reps,
+                           precision=None):
+        """Per-kernel timing of `reps` applications; precision defaults to the
+        handle's own, else a sloppy precision of an FP64 handle."""
         ms = C.c_double()
         msk = np.zeros(8)
         byk = np.zeros(8)
         nk = C.c_int()
-        _chk(lib().dwfgpu_time_matpcdagmatpc(self.h, mass, out.h, inp.h, reps, C.byref(ms),
+        p = _PREC[precision or self.precision]
+        _chk(lib().dwfgpu_time_matpcdagmatpc(self.h, p, mass, out.h, inp.h, reps, C.byref(ms),
                                              msk, byk, C.byref(nk)), "time_MatPcDagMatPc")
         return ms.value, msk[:nk.value], byk[:nk.value]
 
@@ -238,9 +271,10 @@ class _DiracOpGpu:
         _chk(lib().dwfgpu_time_copy(self.h, int(nbytes), reps, C.byref(g)), "time_copy")
         return g.value
 
-    def cost(self):
+    def cost(self, precision=None):
         b, f = C.c_double(), C.c_double()
-        _chk(lib().dwfgpu_matpcdagmatpc_cost(self.h, C.byref(b), C.byref(f)), "cost")
+        p = _PREC[precision or self.precision]
+        _chk(lib().dwfgpu_matpcdagmatpc_cost(self.h, p, C.byref(b), C.byref(f)), "cost")
         return b.value, f.value
 
 

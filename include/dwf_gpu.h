@@ -34,9 +34,17 @@ extern "C" {
 #define DWFGPU_DWF 0    /* DiracOpDwf    (src/util/dirac_op/d_op_dwf/d_op_dwf.C)       */
 #define DWFGPU_MOBIUS 1 /* DiracOpMobius (src/util/dirac_op/d_op_mobius/d_op_mobius.C) */
 
-/* storage precision of the operator's fields on the device */
+/* storage precision of the operator's fields on the device (the handle's
+ * precision in dwfgpu_create, the sloppy precision of dwfgpu_invcg_mixed).
+ * F32, BF16 and I16 store links as their first two rows (the third is
+ * rebuilt as the conjugate cross product times the boundary sign); BF16 and I16
+ * store each (site, slice) spinor as 24 16-bit numbers with one FP32 scale
+ * (BF16: value = bf16 * scale, I16: value = int16 * scale / 32767), and
+ * compute in FP32.                                                          */
 #define DWFGPU_F64 0
 #define DWFGPU_F32 1
+#define DWFGPU_BF16 2
+#define DWFGPU_I16 3
 
 typedef struct dwfgpu_s *dwfgpu_t;
 typedef struct dwfgpu_vec_s *dwfgpu_vec_t;
@@ -57,7 +65,9 @@ typedef struct {
  *   [site x+lx*(y+ly*(z+lz*t))][mu][row][col][re,im] (include/util/lattice.h:327).
  * bc[mu] = 1 multiplies the links U_mu(x) with x_mu = L_mu-1 by -1
  *   (Lattice::BondCond, src/util/lattice/lattice_base/lattice_base.C:5336-5391).
- * mob_b, mob_c are ignored for DWF.  precision: DWFGPU_F64 or DWFGPU_F32.   */
+ * mob_b, mob_c are ignored for DWF.  precision: the storage precision of the
+ * handle's vectors and operator (DWFGPU_F64 ... DWFGPU_I16); host arrays are
+ * FP64 in every case and are converted on the device.                       */
 int dwfgpu_create(const double *gauge_canonical, const int dims[4], int ls, int action,
                   double m5, double a5inv, double mob_b, double mob_c, const int bc[4],
                   int precision, dwfgpu_t *out);
@@ -103,18 +113,39 @@ int dwfgpu_matpcdagmatpc_dev(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_v
 int dwfgpu_invcg_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol,
                      dwfgpu_vec_t src, double *true_res, int *iters);
 
+/* ---- mixed precision ---------------------------------------------------- */
+/* The solve of DiracOp::InvCg (inv_cg.C:84-448: same system, same stop rule
+ * |src - A sol|^2 <= stop_rsd^2 |src|^2 on the FP64 true residual) with the
+ * bulk of the MatPcDagMatPc applications in `sloppy` precision and FP64
+ * reliable updates.  CPS hands this scheme to QUDA under USE_QUDA
+ * (QudaInvert in DiracOpDwf::MatInv, d_op_dwf.C:425-426; sloppy gauge and
+ * spinor precisions and reliable_delta, quda/wilson_quda.C:84,134-137).
+ * A reliable update (sol += sloppy correction, r = src - A sol in FP64)
+ * happens when the sloppy residual norm drops below delta (0 < delta < 1)
+ * times the norm at the previous update.  sloppy = DWFGPU_F64 runs the plain
+ * CG of dwfgpu_invcg.  The handle's API precision must be DWFGPU_F64.
+ * *iters: sloppy iterations + 1; *reliable_updates: their count.           */
+int dwfgpu_invcg_mixed(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, double delta,
+                       double *sol, const double *src, double *true_res, int *iters,
+                       int *reliable_updates);
+int dwfgpu_invcg_mixed_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, double delta,
+                           dwfgpu_vec_t sol, dwfgpu_vec_t src, double *true_res, int *iters,
+                           int *reliable_updates);
+
 /* ---- instrumentation (the role of CPS's print_flops / CGflops) --------- */
 /* Times `reps` MatPcDagMatPc applications on the handle's stream with HIP
- * events around each of its kernel launches.  ms_app: mean per application;
- * ms_kernel[k] (k < *nkernel on return, at most 8): mean per launch of the
- * k-th kernel of one application; bytes_kernel[k]: its algorithmic HBM bytes
- * (inputs read once, outputs written once).                                  */
-int dwfgpu_time_matpcdagmatpc(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_vec_t in,
-                              int reps, double *ms_app, double *ms_kernel,
+ * events around each of its kernel launches.  precision: the handle's API
+ * precision (in and out are used directly), or, on an FP64 handle, a sloppy
+ * precision (in is converted first; out is untouched).  ms_app: mean per
+ * application; ms_kernel[k] (k < *nkernel on return, at most 8): mean per
+ * launch of the k-th kernel of one application; bytes_kernel[k]: its
+ * algorithmic HBM bytes (inputs read once, outputs written once).           */
+int dwfgpu_time_matpcdagmatpc(dwfgpu_t h, int precision, double mass, dwfgpu_vec_t out,
+                              dwfgpu_vec_t in, int reps, double *ms_app, double *ms_kernel,
                               double *bytes_kernel, int *nkernel);
-/* Algorithmic bytes and flops of one MatPcDagMatPc application with the
- * handle's storage precision (see include/roofline.h).                      */
-int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, double *bytes, double *flops);
+/* Algorithmic bytes and flops of one MatPcDagMatPc application in the given
+ * storage precision (see include/roofline.h).                               */
+int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, int precision, double *bytes, double *flops);
 /* Reference copy kernel on the handle's stream: copies `bytes` (rounded
  * down to 16) between two fresh device buffers `reps` times; *gbs receives
  * (read + write bytes) / mean kernel time.                                   */

@@ -1,0 +1,168 @@
+"""GPU: reduced-precision operators and the FP64 reliable-update CG.
+
+Bars
+  * MatPcDagMatPc with 16-bit spinors + FP32 scale per (site, slice) and
+    compressed links, against the FP64 oracle, normwise relative error:
+      BF16 (8-bit significand, unit roundoff 2^-9 = 2e-3, rounded at each of
+      the four stores of one application): < 2e-2
+      I16 (15 bits below the site maximum, 1/65534 = 1.5e-5): < 2e-4
+      F32 storage with compressed links: < 1e-5
+  * the mixed CG must reach the FP64 true residual the plain CG reaches
+    (|src - A sol| <= stop_rsd |src|, checked from an independent FP64
+    MatPcDagMatPc), and its solution must agree with the oracle's solution to
+    the accuracy the stop rule allows (1e-8 relative, as for the plain CG).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cps_veloc_amd import lattice as lat
+from cps_veloc_amd.dwf import BAD_ARG, CgArg, DiracOpDwfGpu, DiracOpMobiusGpu, DwfGpuError
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+M5 = 1.8
+OP_TOL = {"f32": 1e-5, "bf16": 2e-2, "i16": 2e-4}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make(action, dims, ls, precision="f64", eps=0.3, seed=101):
+    U = lat.random_gauge(dims, eps, seed)
+    o = Oracle(dims, ls, action, m5=M5, b=1.5, c=0.5, gauge_canonical=U)
+    if action == "dwf":
+        g = DiracOpDwfGpu(U, dims, ls, m5=M5, precision=precision)
+    else:
+        g = DiracOpMobiusGpu(U, dims, ls, m5=M5, b=1.5, c=0.5, precision=precision)
+    return o, g
+
+
+def src_vec(dims, ls, seed):
+    return lat.to_reals(lat.gaussian_spinor((ls, int(np.prod(dims)) // 2), seed))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("precision", ["f32", "bf16", "i16"])
+@pytest.mark.parametrize("action,dims,ls", [("dwf", (4, 4, 4, 8), 8),
+                                            ("mobius", (4, 4, 4, 8), 8),
+                                            ("mobius", (4, 2, 6, 4), 6)])
+def test_reduced_precision_operator(precision, action, dims, ls):
+    say(f"[gpu] {precision} MatPcDagMatPc {action} {dims} ls={ls}")
+    o, g = make(action, dims, ls, precision)
+    x = src_vec(dims, ls, 31)
+    for mass in (0.04, 0.3):
+        want, dwant = o.matpcdagmatpc(x, mass)
+        got = np.zeros(g.n)
+        d = g.MatPcDagMatPc(got, x, mass)
+        err = rel(got, want)
+        say(f"  mass {mass}: normwise rel err {err:.3e}, dot rel err {abs(d - dwant) / dwant:.3e}")
+        assert err < OP_TOL[precision]
+        assert abs(d - dwant) / dwant < OP_TOL[precision]
+        for dag in (0, 1):
+            wpc = o.matpc(x, mass, dag)
+            gpc = np.zeros(g.n)
+            (g.MatPcDag if dag else g.MatPc)(gpc, x, mass)
+            assert rel(gpc, wpc) < OP_TOL[precision]
+
+
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
+def test_reduced_precision_round_trip(action):
+    """upload -> download through a 16-bit handle: per-site relative error is
+    bounded by the format alone (the scale is the site maximum)."""
+    say(f"[gpu] 16-bit round trip {action}")
+    dims, ls = (4, 4, 4, 4), 4
+    x = src_vec(dims, ls, 37)
+    xs = x.reshape(-1, 24)
+    smax = np.max(np.abs(xs), axis=1, keepdims=True)
+    slack = 1e-6 * smax  # FP32 arithmetic of the scaling itself
+    # bf16 (8 significant bits, round to nearest): |err| <= 2^-8 |x| per component
+    _, g = make(action, dims, ls, "bf16")
+    y = g.vector(x).download().reshape(-1, 24)
+    assert np.all(np.abs(y - xs) <= 2.0 ** -8 * np.abs(xs) + slack)
+    assert not np.array_equal(y, xs)
+    # int16 with scale smax/32767: |err| <= half a step
+    _, g = make(action, dims, ls, "i16")
+    y = g.vector(x).download().reshape(-1, 24)
+    assert np.all(np.abs(y - xs) <= 0.5 * smax / 32767 + slack)
+    # the largest component of every site is exact up to FP32 rounding
+    k = np.argmax(np.abs(xs), axis=1)
+    r = np.arange(xs.shape[0])
+    assert np.all(np.abs(y[r, k] - xs[r, k]) <= slack[:, 0])
+
+
+@pytest.mark.parametrize("sloppy", ["f32", "bf16", "i16"])
+@pytest.mark.parametrize("action,dims,ls,mass", [
+    ("dwf", (8, 8, 8, 8), 8, 0.04),
+    ("mobius", (4, 4, 4, 8), 8, 0.04),
+])
+def test_mixed_cg(sloppy, action, dims, ls, mass):
+    say(f"[gpu] mixed CG sloppy={sloppy} {action} {dims} ls={ls} mf={mass}")
+    o, g = make(action, dims, ls)
+    src = src_vec(dims, ls, 41)
+    osol, oit, otr, _ = o.invcg(src, mass, stop_rsd=1e-10, max_iter=5000)
+    sol = np.zeros(g.n)
+    it, tr, st, nrel = g.InvCgMixed(sol, src, CgArg(mass, 5000, 1e-10), sloppy, 0.1)
+    say(f"  oracle iters={oit}  mixed iters={it} reliable updates={nrel} true_res={tr:.3e}")
+    assert st == 0
+    assert tr <= 1e-10 and nrel >= 1
+    assert it <= 3 * oit
+    # independent check of the true residual with the FP64 operator
+    ax = np.zeros(g.n)
+    g.MatPcDagMatPc(ax, sol, mass)
+    assert np.linalg.norm(src - ax) / np.linalg.norm(src) <= 1.0001e-10
+    assert rel(sol, osol) < 1e-8
+
+
+def test_mixed_cg_f64_is_plain_cg():
+    say("[gpu] mixed CG with an FP64 sloppy operator = plain CG")
+    dims, ls, mass = (4, 4, 4, 8), 8, 0.04
+    o, g = make("mobius", dims, ls)
+    src = src_vec(dims, ls, 43)
+    s1, s2 = np.zeros(g.n), np.zeros(g.n)
+    it1, tr1, st1 = g.InvCg(s1, src, CgArg(mass, 5000, 1e-10))
+    it2, tr2, st2, _ = g.InvCgMixed(s2, src, CgArg(mass, 5000, 1e-10), "f64")
+    assert (it1, st1) == (it2, st2) and tr1 == tr2
+    assert np.array_equal(s1, s2)
+
+
+def test_mixed_cg_maxiter_and_args():
+    say("[gpu] mixed CG: MAXITER and argument checks")
+    dims, ls, mass = (4, 4, 4, 4), 4, 0.1
+    o, g = make("dwf", dims, ls)
+    src = src_vec(dims, ls, 47)
+    sol = np.zeros(g.n)
+    it, tr, st, _ = g.InvCgMixed(sol, src, CgArg(mass, 4, 1e-14), "bf16", 0.1)
+    assert st == 2 and it == 4
+    ax = np.zeros(g.n)
+    g.MatPcDagMatPc(ax, sol, mass)
+    assert abs(np.linalg.norm(src - ax) / np.linalg.norm(src) - tr) < 1e-6 * tr
+    for delta in (0.0, 1.0, -0.5):
+        with pytest.raises(DwfGpuError) as e:
+            g.InvCgMixed(np.zeros(g.n), src, CgArg(mass, 10, 1e-8), "f32", delta)
+        assert e.value.code == BAD_ARG
+    _, g32 = make("dwf", dims, ls, "f32")
+    with pytest.raises(DwfGpuError) as e:
+        g32.InvCgMixed(np.zeros(g32.n), src, CgArg(mass, 10, 1e-8), "bf16", 0.1)
+    assert e.value.code == BAD_ARG
+
+
+def test_sloppy_timing_on_fp64_handle():
+    say("[gpu] timing of sloppy operators on an FP64 handle")
+    dims, ls = (4, 4, 4, 8), 8
+    o, g = make("mobius", dims, ls)
+    x = src_vec(dims, ls, 53)
+    vin, vout = g.vector(x), g.vector()
+    for p in ("f64", "f32", "bf16", "i16"):
+        ms, msk, byk = g.time_MatPcDagMatPc(vout, vin, 0.04, 2, precision=p)
+        b, _ = g.cost(p)
+        assert ms > 0 and len(msk) == 5
+        assert abs(sum(byk[:4]) - b) < 1e-6 * b
+    assert np.array_equal(vin.download(), x)
