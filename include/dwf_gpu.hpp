@@ -1,0 +1,143 @@
+// dwf_gpu.hpp -- header-only C++ facade over the C ABI of dwf_gpu.h with the
+// member functions of CPS's DiracOpDwf / DiracOpMobius
+// (include/util/dirac_op/d_op_dwf.h:26-140, include/util/dirac_op.h:1796-1900),
+// so that FmatEvlInv-style host code swaps one type.  Vectors are CPS
+// half (or, for MatInv, full) vectors of FP64 reals in CPS's own order;
+// CPS's Vector* maps to double* when Float is double (its default).
+//
+// Errors: CPS aborts through ERR.General; the facade throws dwfgpu::Error
+// carrying the C status and message.  MAXITER is not an error (CPS warns):
+// the iteration count and true residual are returned as usual.
+#ifndef DWF_GPU_HPP
+#define DWF_GPU_HPP
+
+#include <stdexcept>
+#include <string>
+
+#include "dwf_gpu.h"
+
+namespace dwfgpu {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string& where)
+      : std::runtime_error(where + " failed with status " + std::to_string(code) + ": " +
+                           dwfgpu_last_error()),
+        code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+inline int check(int code, const char* where, bool maxiter_ok = false) {
+  if (code == DWFGPU_OK || (maxiter_ok && code == DWFGPU_MAXITER)) return code;
+  throw Error(code, where);
+}
+
+// Mirror of CgArg (include/input/cg_arg.x:3-28), the fields InvCg reads.
+struct CgArg {
+  double mass = 0.0;
+  int max_num_iter = 10000;
+  double stop_rsd = 1e-10;
+  double true_rsd = 1e-10;
+  dwfgpu_cg_arg c() const { return dwfgpu_cg_arg{mass, max_num_iter, stop_rsd, true_rsd}; }
+};
+
+// Lattice and action parameters CPS reads from GJP (GJP.NodeSites, SnodeSites,
+// DwfHeight, DwfA5Inv, Mobius_b/c, Bc).
+struct Params {
+  int dims[4] = {4, 4, 4, 4};
+  int ls = 4;
+  double m5 = 1.8;
+  double a5inv = 1.0;
+  double mobius_b = 1.5;
+  double mobius_c = 0.5;
+  int bc[4] = {0, 0, 0, 1};  // 1: anti-periodic links at x_mu = L_mu - 1
+  int precision = DWFGPU_F64;
+};
+
+class DiracOpGpu {
+ public:
+  DiracOpGpu(const DiracOpGpu&) = delete;
+  DiracOpGpu& operator=(const DiracOpGpu&) = delete;
+  virtual ~DiracOpGpu() {
+    if (h_) dwfgpu_destroy(h_);
+  }
+
+  // number of reals in one half vector (CPS: RitzLatSize / f_size_cb)
+  size_t HalfSize() const { return dwfgpu_half_size(h_); }
+  dwfgpu_t handle() const { return h_; }
+
+  // DiracOp::MatPcDagMatPc(out, in, dot_prd)     dirac_op.h:259
+  void MatPcDagMatPc(double* out, const double* in, double* dot_prd = nullptr) {
+    check(dwfgpu_matpcdagmatpc(h_, arg_.mass, out, in, dot_prd), "MatPcDagMatPc");
+  }
+  // DiracOp::MatPc / MatPcDag                     d_op_dwf.C:245-300
+  void MatPc(double* out, const double* in) {
+    check(dwfgpu_matpc(h_, arg_.mass, out, in, 0), "MatPc");
+  }
+  void MatPcDag(double* out, const double* in) {
+    check(dwfgpu_matpc(h_, arg_.mass, out, in, 1), "MatPcDag");
+  }
+  // DiracOp::Dslash(out, in, cb, dag)             d_op_dwf.C:210-236
+  void Dslash(double* out, const double* in, int cb, int dag) {
+    check(dwfgpu_dslash(h_, arg_.mass, out, in, cb, dag), "Dslash");
+  }
+  // DiracOp::InvCg(out, in, true_res): returns CPS's iteration count
+  int InvCg(double* out, const double* in, double* true_res = nullptr) {
+    int iters = 0;
+    const dwfgpu_cg_arg a = arg_.c();
+    check(dwfgpu_invcg(h_, &a, out, in, true_res, &iters), "InvCg", true);
+    return iters;
+  }
+  // DiracOpDwf::MatInv / DiracOpMobius::MatInv (PRESERVE_YES)
+  int MatInv(double* out, const double* in, double* true_res = nullptr) {
+    int iters = 0;
+    const dwfgpu_cg_arg a = arg_.c();
+    check(dwfgpu_matinv(h_, &a, out, in, true_res, &iters), "MatInv", true);
+    return iters;
+  }
+  // InvCg's solve with a sloppy operator and FP64 reliable updates
+  int InvCgMixed(double* out, const double* in, int sloppy, double delta = 0.1,
+                 double* true_res = nullptr, int* reliable_updates = nullptr) {
+    int iters = 0, nrel = 0;
+    const dwfgpu_cg_arg a = arg_.c();
+    check(dwfgpu_invcg_mixed(h_, &a, sloppy, delta, out, in, true_res, &iters, &nrel),
+          "InvCgMixed", true);
+    if (reliable_updates) *reliable_updates = nrel;
+    return iters;
+  }
+  // DiracOp::DiracArg(CgArg*): the mass and CG controls of later calls
+  void DiracArg(const CgArg& arg) { arg_ = arg; }
+
+ protected:
+  DiracOpGpu(const double* gauge_canonical, const Params& p, const CgArg& arg, int action)
+      : arg_(arg) {
+    check(dwfgpu_create(gauge_canonical, p.dims, p.ls, action, p.m5, p.a5inv, p.mobius_b,
+                        p.mobius_c, p.bc, p.precision, &h_),
+          "dwfgpu_create");
+  }
+
+ private:
+  dwfgpu_t h_ = nullptr;
+  CgArg arg_;
+};
+
+// DiracOpDwf (src/util/dirac_op/d_op_dwf/d_op_dwf.C)
+class DiracOpDwf : public DiracOpGpu {
+ public:
+  DiracOpDwf(const double* gauge_canonical, const Params& p, const CgArg& arg)
+      : DiracOpGpu(gauge_canonical, p, arg, DWFGPU_DWF) {}
+};
+
+// DiracOpMobius (src/util/dirac_op/d_op_mobius/d_op_mobius.C)
+class DiracOpMobius : public DiracOpGpu {
+ public:
+  DiracOpMobius(const double* gauge_canonical, const Params& p, const CgArg& arg)
+      : DiracOpGpu(gauge_canonical, p, arg, DWFGPU_MOBIUS) {}
+};
+
+}  // namespace dwfgpu
+
+#endif

@@ -18,7 +18,30 @@
  *   Mobius half vector [s][cbsite][24], every slice holds the same 4D parity
  *                      (d_op_zmobius/noarch/zmobius_dslash_4.C:109-121).
  *
- * Parity pinning of each function is listed in oracle/README.md.
+ * CPS cannot be compiled here from its own sources without a configure-
+ * generated config.h and stand-ins for the QUDA header dirac_op.h includes,
+ * and the reference holds no vectors for this path, so each function is
+ * pinned by mathematics (tests/test_oracle.py against the independent numpy
+ * operators of tests/textbook.py):
+ *   orc_gauge_canonical_to_wilson, orc_fermion_extract_cb/insert_cb
+ *       round trips; BondCond signs through test_boundary_conditions_vs_textbook
+ *   orc_wilson_dslash       textbook hop on 3 lattice shapes x cb x dag, three
+ *                           boundary conditions, free-field plane-wave eigenvalue
+ *   orc_dwf_dslash_5_plus, orc_dwf_dslash
+ *                           dense textbook Shamir operator, both dag
+ *   orc_dwf_m/mdag/mdagm, orc_matpc, orc_matpcdagmatpc (DWF)
+ *                           dense even-odd Schur complement of the textbook
+ *                           operator; adjoint identity; MdagM = Mdag M
+ *   orc_zmobius_kappa_dslash_5_plus, orc_zmobius_dslash_4, orc_mobius_coefficients,
+ *   orc_mobius_m/mdag/mdagm (Mobius)
+ *                           dense Schur complement of the textbook Mobius
+ *                           operator for (b,c) = (1.5,0.5), (1,0), (2,1);
+ *                           textbook (b + c D5) D4; adjoint identity
+ *   orc_zmobius_m5inv       M5 M5^-1 = 1 and the dense inverse, Ls 4/8/12, dag 0/1
+ *   orc_invcg, orc_norm_sq  exactly k+1 iterations on a source spanned by k
+ *                           eigenvectors; solution vs dense solve; MAXITER and
+ *                           converged-guess edges
+ *   orc_matinv              full textbook system solved for both actions
  */
 #ifndef CPS_ORACLE_H
 #define CPS_ORACLE_H
