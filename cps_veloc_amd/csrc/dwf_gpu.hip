@@ -111,11 +111,11 @@ constexpr int pidx<PI16>() { return DWFGPU_I16; }
 
 template <class P>
 dwfk::Fld<P> fld(const DVec& v) {
-  return dwfk::Fld<P>{(typename P::S*)v.d, v.sc};
+  return dwfk::Fld<P>{(typename P::C*)v.d, v.sc};
 }
 template <class P>
 dwfk::CFld<P> cfld(const DVec& v) {
-  return dwfk::CFld<P>{(const typename P::S*)v.d, v.sc};
+  return dwfk::CFld<P>{(const typename P::C*)v.d, v.sc};
 }
 
 // bytes of one half vector / of all links (once) in policy P
@@ -211,9 +211,10 @@ MobArgs mob_args(const dwfgpu_s* h, double mass, int dag) {
 size_t col_lds_bytes(const dwfgpu_s* h, size_t creal) {
   return (size_t)h->ns * (h->ls * 12 + 1) * 2 * creal;
 }
-// LDS of the staged links: nsets parities x 8 directions x ns sites x 9
+// LDS of the staged links and neighbour indices: nsets parities x 8
+// directions x ns sites x (9 complex + 1 int)
 size_t link_lds_bytes(const dwfgpu_s* h, int nsets, size_t creal) {
-  return (size_t)nsets * 8 * h->ns * 9 * 2 * creal;
+  return (size_t)nsets * 8 * h->ns * (9 * 2 * creal + 4);
 }
 // largest LDS request of any kernel of this handle (FP64 compute is the widest)
 size_t max_lds_bytes(const dwfgpu_s* h) {
@@ -276,6 +277,22 @@ int ensure_work(dwfgpu_s* h, int count) {
 // ---------------------------------------------------------------------------
 // reductions
 // ---------------------------------------------------------------------------
+// sum of nparts partials into the device scalar dscal[slot] (no host sync)
+int reduce_to(dwfgpu_s* h, int nparts, int slot) {
+  hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(1024), 0, h->st, h->partial, nparts,
+                     h->dscal + slot);
+  HIPCHK(hipGetLastError());
+  return DWFGPU_OK;
+}
+
+// copy dscal[0..n) to the host and wait
+int fetch_scalars(dwfgpu_s* h, int n, double* out) {
+  HIPCHK(hipMemcpyAsync(h->hscal, h->dscal, n * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int k = 0; k < n; ++k) out[k] = h->hscal[k];
+  return DWFGPU_OK;
+}
+
 int fetch_sum(dwfgpu_s* h, int nparts, double* out) {
   hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(1024), 0, h->st, h->partial, nparts,
                      h->dscal);
@@ -305,7 +322,7 @@ struct Ops {
 #define DWF_LAUNCH(DAG, EPI, NORM)                                                           \
   hipLaunchKernelGGL((dwfk::k_dwf_dslash<P, DAG, EPI, NORM>), grid, block, lds, h->st,       \
                      fld<P>(out), cfld<P>(in), u, cfld<P>(x0), h->partial, h->g, cb_in, two_a5, \
-                     nm, alpha)
+                     nm, alpha, dwfk::CgEpi<P, P>{})
     if (epi == 0) {
       if (dag) DWF_LAUNCH(1, 0, false); else DWF_LAUNCH(0, 0, false);
     } else if (norm) {
@@ -328,7 +345,7 @@ struct Ops {
     const size_t lds = link_lds_bytes(h, 1, sizeof(R)) + (col ? col_lds_bytes(h, sizeof(R)) : 0);
 #define MOB_LAUNCH(DAG, EPI, FL)                                                              \
   hipLaunchKernelGGL((dwfk::k_mobius<P, DAG, EPI, FL>), grid, block, lds, h->st, fld<P>(out), \
-                     cfld<P>(in), cfld<P>(psi), u, h->partial, h->g, p_out, a)
+                     cfld<P>(in), cfld<P>(psi), u, h->partial, h->g, p_out, a, dwfk::CgEpi<P, P>{})
     using dwfk::FL_NORM;
     using dwfk::FL_POST_T;
     using dwfk::FL_PRE_T;
@@ -403,6 +420,48 @@ struct Ops {
     return DWFGPU_OK;
   }
 
+  // One CG step around MatPcDagMatPc (inv_cg.C:318-349) with the vector
+  // updates fused into the last kernel: with rr = dscal[0] on entry,
+  // d = <p, A p> -> dscal[1], a = rr / d, x += a p, r -= a A p, and
+  // |r|^2 -> dscal[0].  *rr_new, *d receive the two scalars.
+  template <class PX>
+  static int mdagm_cg(dwfgpu_s* h, double mass, const DVec& p, const DVec& r, const DVec& x,
+                      double* rr_new, double* d) {
+    const DVec& te = F(h).te;
+    const DVec& t1 = F(h).t1;
+    const dwfk::CgEpi<P, PX> cg{fld<P>(r), cfld<P>(p), fld<PX>(x), h->dscal};
+    const dim3 grid(h->grid_hop), block(h->ns, h->ls);
+    const S* u = (const S*)F(h).U;
+    if (h->action == DWFGPU_DWF) {
+      const double mk2 = -dwf_kappa(h) * dwf_kappa(h);
+      TRY(dwf(h, mass, te, p, 1, 0, 0, DVec(), 0.0, false));
+      TRY(dwf(h, mass, t1, te, 0, 0, 1, p, mk2, true));
+      TRY(reduce_to(h, h->grid_hop, 1));
+      TRY(dwf(h, mass, te, t1, 1, 1, 0, DVec(), 0.0, false));
+      hipLaunchKernelGGL((dwfk::k_dwf_dslash<P, 1, 1, false, true, PX>), grid, block,
+                         link_lds_bytes(h, 2, sizeof(R)), h->st, fld<P>(te), cfld<P>(te), u,
+                         cfld<P>(t1), h->partial, h->g, 0, 2.0 * h->a5inv,
+                         -2.0 * mass * h->a5inv, mk2, cg);
+    } else {
+      TRY(mob_first(h, mass, te, p, 0));
+      TRY(mob_second(h, mass, t1, te, p, 0, true));
+      TRY(reduce_to(h, h->grid_hop, 1));
+      TRY(mob_first(h, mass, te, t1, 1));
+      const MobArgs a = mob_args(h, mass, 1);
+      const size_t lds = link_lds_bytes(h, 1, sizeof(R)) + col_lds_bytes(h, sizeof(R));
+      hipLaunchKernelGGL((dwfk::k_mobius<P, 1, dwfk::EPI_SCHUR, dwfk::FL_CG, PX>), grid, block,
+                         lds, h->st, fld<P>(te), cfld<P>(te), cfld<P>(t1), u, h->partial, h->g, 1,
+                         a, cg);
+    }
+    HIPCHK(hipGetLastError());
+    TRY(reduce_to(h, h->grid_hop, 0));
+    double sc[2];
+    TRY(fetch_scalars(h, 2, sc));
+    *rr_new = sc[0];
+    *d = sc[1];
+    return DWFGPU_OK;
+  }
+
   static int dslash(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int cb, int dag) {
     if (h->action == DWFGPU_DWF) return dwf(h, mass, out, in, cb, dag, 0, DVec(), 0.0, false);
     return mob(h, mass, out, in, DVec(), 1 - cb, dag, dwfk::EPI_PLAIN, 0);
@@ -470,20 +529,16 @@ struct Ops {
     int max_itr = arg->max_num_iter - 1;
     if (rr <= stp_cnd) max_itr = 0;
     int status = DWFGPU_OK;
+    // dscal[0] holds rr (fetch_sum above); each step replaces it by the new rr
     for (int k = 0; k < max_itr; ++k) {
       itr = itr + 1;
       const double rr_prv = rr;
       double d;
-      TRY(mdagm(h, mass, mmp, dir, &d));
-      if (d == 0.0) {
+      TRY(mdagm_cg<P>(h, mass, dir, res, sol, &rr, &d));
+      if (d == 0.0) {  // a = 0 was used: sol and res are unchanged
         status = DWFGPU_BREAKDOWN;
         break;
       }
-      const double a = rr_prv / d;
-      hipLaunchKernelGGL((dwfk::k_cg_update<P, P>), blas_grid(), blas_block(), 0, h->st,
-                         fld<P>(sol), fld<P>(res), cfld<P>(dir), cfld<P>(mmp), a, h->g, h->partial);
-      HIPCHK(hipGetLastError());
-      TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
       if (rr <= stp_cnd) break;
       const double b = rr / rr_prv;
       TRY(axpby(h, dir, res, 1.0, b));  // dir = res + b dir
@@ -578,7 +633,6 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   const DVec& ax = h->f[DWFGPU_F64].work[1];
   const DVec& rs = h->f[pidx<PS>()].work[0];
   const DVec& ps = h->f[pidx<PS>()].work[1];
-  const DVec& aps = h->f[pidx<PS>()].work[2];
   const DVec& xs = h->f[DWFGPU_F32].work[3];
   const double mass = arg->mass;
   double src_norm_sq;
@@ -600,18 +654,13 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   bool converged = rr <= stp_cnd;
   while (!converged && itr < max_itr) {
     itr = itr + 1;
-    double d;
-    TRY(L::mdagm(h, mass, aps, ps, &d));
+    double d, rr_new;
+    // dscal[0] holds rr; the fused step updates xs, rs and dscal[0]
+    TRY(L::template mdagm_cg<P32>(h, mass, ps, rs, xs, &rr_new, &d));
     if (d == 0.0) {
       status = DWFGPU_BREAKDOWN;
       break;
     }
-    const double a = rr / d;
-    hipLaunchKernelGGL((dwfk::k_cg_update<PS, P32>), blas_grid(), blas_block(), 0, h->st,
-                       fld<P32>(xs), fld<PS>(rs), cfld<PS>(ps), cfld<PS>(aps), a, h->g, h->partial);
-    HIPCHK(hipGetLastError());
-    double rr_new;
-    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr_new));
     if (rr_new <= stp_cnd || rr_new < delta * delta * rr_update) {
       // reliable update in FP64
       hipLaunchKernelGGL((dwfk::k_accumulate<P32>), blas_grid(), blas_block(), 0, h->st,

@@ -11,8 +11,10 @@
 // Compressed links are rebuilt as row2 = conj(row0 x row1) times the
 // anti-periodic boundary sign when they are staged in LDS.
 //
-// Device layout (structure of arrays):
-//   spinor data   d[(s*12 + c)*vh + i]   c = spin*3 + colour, one complex per word
+// Device layout (structure of arrays of 16-byte chunks):
+//   spinor data   d[(s*NCH + k)*vh + i]  chunk k holds the complex components
+//                 c = spin*3 + colour in [k*CPC, (k+1)*CPC): CPC = 1 (P64),
+//                 2 (P32), 4 (16-bit), NCH = 12/CPC; one 16-byte load each
 //   spinor scale  sc[s*vh + i]           (scaled policies only)
 //   links         U[((mu*2 + parity)*NE + e)*vh + i]   NE = 9 (P64) or 6
 //                 e = row*3 + col of canonical U(row, col), boundary signs folded in
@@ -52,6 +54,43 @@ template <typename R>
 __device__ __forceinline__ cx<R> cmulc(cx<R> a, cx<R> b) {
   return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
 }
+// U h and U^dag h for one colour vector h (3 complex), as chains of fused
+// multiply-adds (4 per complex product, no separate sums)
+template <typename R>
+__device__ __forceinline__ cx<R> row_dot(cx<R> a0, cx<R> a1, cx<R> a2, cx<R> b0, cx<R> b1,
+                                         cx<R> b2) {
+  R re = a0.re * b0.re;
+  re = fma(-a0.im, b0.im, re);
+  re = fma(a1.re, b1.re, re);
+  re = fma(-a1.im, b1.im, re);
+  re = fma(a2.re, b2.re, re);
+  re = fma(-a2.im, b2.im, re);
+  R im = a0.re * b0.im;
+  im = fma(a0.im, b0.re, im);
+  im = fma(a1.re, b1.im, im);
+  im = fma(a1.im, b1.re, im);
+  im = fma(a2.re, b2.im, im);
+  im = fma(a2.im, b2.re, im);
+  return {re, im};
+}
+// conj(a0) b0 + conj(a1) b1 + conj(a2) b2
+template <typename R>
+__device__ __forceinline__ cx<R> row_dotc(cx<R> a0, cx<R> a1, cx<R> a2, cx<R> b0, cx<R> b1,
+                                          cx<R> b2) {
+  R re = a0.re * b0.re;
+  re = fma(a0.im, b0.im, re);
+  re = fma(a1.re, b1.re, re);
+  re = fma(a1.im, b1.im, re);
+  re = fma(a2.re, b2.re, re);
+  re = fma(a2.im, b2.im, re);
+  R im = a0.re * b0.im;
+  im = fma(-a0.im, b0.re, im);
+  im = fma(a1.re, b1.im, im);
+  im = fma(-a1.im, b1.re, im);
+  im = fma(a2.re, b2.im, im);
+  im = fma(-a2.im, b2.re, im);
+  return {re, im};
+}
 // sg * i * a  (sg = +-1)
 template <int SG, typename R>
 __device__ __forceinline__ cx<R> sgi(cx<R> a) {
@@ -65,30 +104,42 @@ __device__ __forceinline__ cx<R> sgr(cx<R> a) {
 // ---------------------------------------------------------------------------
 // storage policies
 // ---------------------------------------------------------------------------
+// R: compute type; S: one stored complex number (links use S);
+// C: the 16-byte chunk of CPC complex numbers spinors are stored in
 struct P64 {
   typedef double R;
   typedef cx<double> S;
+  typedef cx<double> C;
   static constexpr bool kScaled = false;
   static constexpr int kLinkElems = 9;
+  static constexpr int kCPC = 1;
 };
 struct P32 {
   typedef float R;
   typedef cx<float> S;
+  typedef float4 C;
   static constexpr bool kScaled = false;
   static constexpr int kLinkElems = 6;
+  static constexpr int kCPC = 2;
 };
 struct PBF16 {
   typedef float R;
   typedef uint32_t S;  // re in the low half, im in the high half
+  typedef uint4 C;
   static constexpr bool kScaled = true;
   static constexpr int kLinkElems = 6;
+  static constexpr int kCPC = 4;
 };
 struct PI16 {
   typedef float R;
   typedef uint32_t S;
+  typedef uint4 C;
   static constexpr bool kScaled = true;
   static constexpr int kLinkElems = 6;
+  static constexpr int kCPC = 4;
 };
+template <class P>
+constexpr int nch() { return 12 / P::kCPC; }
 
 // scale stored per (site, slice) from the largest |component| m of the site
 template <class P>
@@ -145,45 +196,74 @@ __device__ __forceinline__ float link_scale<PI16>() { return 1.0f / 32767.0f; }
 // a spinor field: data + per-(site,slice) scales (scaled policies only)
 template <class P>
 struct Fld {
-  typename P::S* d;
+  typename P::C* d;
   float* sc;
 };
 template <class P>
 struct CFld {
-  const typename P::S* d;
+  const typename P::C* d;
   const float* sc;
 };
 
-// complex component c of site n in the slice starting at (d, sc)
+// chunk <-> CPC complex numbers
 template <class P>
-__device__ __forceinline__ cx<typename P::R> ld(const typename P::S* __restrict__ d,
-                                                const float* __restrict__ sc, int vh, int c,
-                                                int n, float s) {
-  (void)sc;
-  return Pack<P>::unpack(d[(size_t)c * vh + n], s);
+__device__ __forceinline__ void chunk_unpack(const typename P::C& w, float sc,
+                                             cx<typename P::R>* v) {
+  if constexpr (P::kCPC == 1) {
+    v[0] = w;
+  } else if constexpr (P::kCPC == 2) {
+    v[0] = cx<float>{w.x, w.y};
+    v[1] = cx<float>{w.z, w.w};
+  } else {
+    v[0] = Pack<P>::unpack(w.x, sc);
+    v[1] = Pack<P>::unpack(w.y, sc);
+    v[2] = Pack<P>::unpack(w.z, sc);
+    v[3] = Pack<P>::unpack(w.w, sc);
+  }
 }
+template <class P>
+__device__ __forceinline__ typename P::C chunk_pack(const cx<typename P::R>* v, float inv) {
+  if constexpr (P::kCPC == 1) {
+    return v[0];
+  } else if constexpr (P::kCPC == 2) {
+    return float4{v[0].re, v[0].im, v[1].re, v[1].im};
+  } else {
+    return uint4{Pack<P>::pack(v[0], inv), Pack<P>::pack(v[1], inv), Pack<P>::pack(v[2], inv),
+                 Pack<P>::pack(v[3], inv)};
+  }
+}
+
 template <class P>
 __device__ __forceinline__ float ld_scale(const float* __restrict__ sc, int n) {
   return P::kScaled ? sc[n] : 1.0f;
 }
 
+// the 12 complex components of site n of one slice (d, sc: the slice's base).
+// Chunks whose components the caller does not use are dead loads and vanish.
+template <class P>
+__device__ __forceinline__ void ld_spinor(const typename P::C* __restrict__ d,
+                                          const float* __restrict__ sc, int vh, int n,
+                                          cx<typename P::R>* v) {
+  const float s = ld_scale<P>(sc, n);
+#pragma unroll
+  for (int k = 0; k < nch<P>(); ++k) chunk_unpack<P>(d[(size_t)k * vh + n], s, v + k * P::kCPC);
+}
+
 // store the 12 complex components of site n of a slice
 template <class P>
-__device__ __forceinline__ void st_site(typename P::S* __restrict__ d, float* __restrict__ sc,
+__device__ __forceinline__ void st_site(typename P::C* __restrict__ d, float* __restrict__ sc,
                                         int vh, int n, const cx<typename P::R>* v) {
+  float inv = 1.0f;
   if constexpr (P::kScaled) {
     float m = 0.0f;
 #pragma unroll
     for (int c = 0; c < 12; ++c) m = fmaxf(m, fmaxf(fabsf((float)v[c].re), fabsf((float)v[c].im)));
     const float s = scale_of<P>(m);
-    const float inv = m > 0.0f ? 1.0f / s : 0.0f;
+    inv = m > 0.0f ? 1.0f / s : 0.0f;
     sc[n] = s;
-#pragma unroll
-    for (int c = 0; c < 12; ++c) d[(size_t)c * vh + n] = Pack<P>::pack(v[c], inv);
-  } else {
-#pragma unroll
-    for (int c = 0; c < 12; ++c) d[(size_t)c * vh + n] = Pack<P>::pack(v[c], 1.0f);
   }
+#pragma unroll
+  for (int k = 0; k < nch<P>(); ++k) d[(size_t)k * vh + n] = chunk_pack<P>(v + k * P::kCPC, inv);
 }
 
 // ---------------------------------------------------------------------------
@@ -318,15 +398,19 @@ __device__ __forceinline__ void reconstruct(cx<R>* a, const cx<R>* r) {
 }
 
 // plain load of the input spinor at site n of one slice
+// A loader fills p with the spinor at site n; for 16-bit policies it leaves
+// out the per-site scale and returns it, so the caller applies it to the
+// spin-projected half spinor (6 instead of 12 complex products).
 template <class P>
 struct PlainLoader {
-  const typename P::S* __restrict__ d;
+  const typename P::C* __restrict__ d;
   const float* __restrict__ sc;
   int vh;
-  __device__ __forceinline__ void operator()(int n, cx<typename P::R>* p) const {
-    const float s = ld_scale<P>(sc, n);
+  __device__ __forceinline__ float operator()(int n, cx<typename P::R>* p) const {
 #pragma unroll
-    for (int c = 0; c < 12; ++c) p[c] = ld<P>(d, sc, vh, c, n, s);
+    for (int k = 0; k < nch<P>(); ++k)
+      chunk_unpack<P>(d[(size_t)k * vh + n], 1.0f, p + k * P::kCPC);
+    return ld_scale<P>(sc, n);
   }
 };
 
@@ -335,50 +419,53 @@ struct PlainLoader {
 template <class P>
 struct MobTLoader {
   typedef typename P::R R;
-  const typename P::S* __restrict__ cur;
-  const typename P::S* __restrict__ prv;
-  const typename P::S* __restrict__ nxt;
+  const typename P::C* __restrict__ cur;
+  const typename P::C* __restrict__ prv;
+  const typename P::C* __restrict__ nxt;
   const float* __restrict__ sc_cur;
   const float* __restrict__ sc_prv;
   const float* __restrict__ sc_nxt;
   int vh;
   R b, cm, cp;
-  __device__ __forceinline__ void operator()(int n, cx<R>* p) const {
-    const float s0 = ld_scale<P>(sc_cur, n), sm = ld_scale<P>(sc_prv, n),
-                sp = ld_scale<P>(sc_nxt, n);
+  __device__ __forceinline__ float operator()(int n, cx<R>* p) const {
+    cx<R> vm[12], vp[12];
+    ld_spinor<P>(cur, sc_cur, vh, n, p);
+    ld_spinor<P>(prv, sc_prv, vh, n, vm);
+    ld_spinor<P>(nxt, sc_nxt, vh, n, vp);
 #pragma unroll
-    for (int c = 0; c < 6; ++c)
-      p[c] = b * ld<P>(cur, sc_cur, vh, c, n, s0) + cm * ld<P>(prv, sc_prv, vh, c, n, sm);
+    for (int c = 0; c < 6; ++c) p[c] = b * p[c] + cm * vm[c];
 #pragma unroll
-    for (int c = 6; c < 12; ++c)
-      p[c] = b * ld<P>(cur, sc_cur, vh, c, n, s0) + cp * ld<P>(nxt, sc_nxt, vh, c, n, sp);
+    for (int c = 6; c < 12; ++c) p[c] = b * p[c] + cp * vp[c];
+    return 1.0f;
   }
 };
 
-// site index of the link U_MU(x) (bwd: U_MU(x - MU)) of the site with
-// coordinates c, and whether it carries the anti-periodic boundary sign
+// For the site with coordinates c: the index of its neighbour x + MU (fwd)
+// or x - MU (bwd) on the other parity, whether the link U_MU(x) (fwd) or
+// U_MU(x - MU) (bwd) carries the anti-periodic boundary sign, and the index of
+// that link's site (x itself, or the backward neighbour).
 template <int MU>
-__device__ __forceinline__ void link_site(const Geom& g, const int* c, bool bwd, int* site,
-                                          bool* neg) {
+__device__ __forceinline__ void link_site(const Geom& g, const int* c, int i, bool bwd,
+                                          int* site, bool* neg, int* nbr) {
   const int L = MU == 0 ? g.lx : MU == 1 ? g.ly : MU == 2 ? g.lz : g.lt;
   const int x = c[MU];
   const bool bc = (g.bcmask >> MU) & 1;
-  if (!bwd) {
-    *neg = bc && x == L - 1;
-    return;
-  }
-  *neg = bc && x == 0;
-  const int m = x == 0 ? L - 1 : x - 1;
-  *site = site_index(g, (MU == 0 ? m : c[0]) >> 1, MU == 1 ? m : c[1], MU == 2 ? m : c[2],
-                     MU == 3 ? m : c[3]);
+  *neg = bc && (bwd ? x == 0 : x == L - 1);
+  const int m = bwd ? (x == 0 ? L - 1 : x - 1) : (x == L - 1 ? 0 : x + 1);
+  *nbr = site_index(g, (MU == 0 ? m : c[0]) >> 1, MU == 1 ? m : c[1], MU == 2 ? m : c[2],
+                    MU == 3 ? m : c[3]);
+  *site = bwd ? *nbr : i;
 }
 
 // Links of the work-group's sites, staged once in LDS (compute type) and
 // shared by all Ls slices: lu[((set*8 + d)*ns + site)*9 + e], d < 4 the
 // forward link U_d(x), d >= 4 the backward link U_{d-4}(x - d) of the output
 // site x; `set` distinguishes the two output parities of a DWF work-group.
+// lnb[(set*8 + d)*ns + site] receives the index of the neighbour x + d
+// (d < 4) or x - (d-4) that the hop reads, so no slice recomputes it.
 template <class P>
 __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
+                                            int* __restrict__ lnb,
                                             const typename P::S* __restrict__ U, const Geom& g,
                                             int blk, int ns, int nsets, int p_first) {
   typedef typename P::R R;
@@ -393,16 +480,17 @@ __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
     const int mu = d & 3;
     int c[4];
     site_coords(g, i, p, c);
-    int site = i;
+    int site = i, nbr = i;
     bool neg = false;
     // one case per direction: a lane-dependent index into {lx, ly, lz, lt}
     // would be lowered to a private (scratch) array
     switch (mu) {
-      case 0: link_site<0>(g, c, d >= 4, &site, &neg); break;
-      case 1: link_site<1>(g, c, d >= 4, &site, &neg); break;
-      case 2: link_site<2>(g, c, d >= 4, &site, &neg); break;
-      default: link_site<3>(g, c, d >= 4, &site, &neg); break;
+      case 0: link_site<0>(g, c, i, d >= 4, &site, &neg, &nbr); break;
+      case 1: link_site<1>(g, c, i, d >= 4, &site, &neg, &nbr); break;
+      case 2: link_site<2>(g, c, i, d >= 4, &site, &neg, &nbr); break;
+      default: link_site<3>(g, c, i, d >= 4, &site, &neg, &nbr); break;
     }
+    lnb[w] = nbr;
     const int par = d >= 4 ? 1 - p : p;  // U_mu(x - mu) lives on the other parity
     const typename P::S* src = U + (size_t)((mu * 2 + par) * NE) * g.vh + site;
     cx<R>* dst = lu + (size_t)w * 9;
@@ -430,8 +518,10 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
   constexpr int SF = DAG ? 1 : -1;  // forward: 1 - s gamma, s = +1 (dag 0) / -1 (dag 1)
   cx<R> psi[12], h[6], r[6];
   {
-    ld(nb.fwd[MU], psi);
+    const R sc = (R)ld(nb.fwd[MU], psi);
     project<MU, SF>(psi, h);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h[k] = sc * h[k];
     cx<R> m[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) m[e] = lf[e];
@@ -439,13 +529,15 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
     for (int k = 0; k < 2; ++k)
 #pragma unroll
       for (int row = 0; row < 3; ++row)
-        r[3 * k + row] = cmul(m[row * 3 + 0], h[3 * k + 0]) + cmul(m[row * 3 + 1], h[3 * k + 1]) +
-                         cmul(m[row * 3 + 2], h[3 * k + 2]);
+        r[3 * k + row] = row_dot(m[row * 3 + 0], m[row * 3 + 1], m[row * 3 + 2], h[3 * k + 0],
+                                 h[3 * k + 1], h[3 * k + 2]);
     reconstruct<MU, SF>(acc, r);
   }
   {
-    ld(nb.bwd[MU], psi);
+    const R sc = (R)ld(nb.bwd[MU], psi);
     project<MU, -SF>(psi, h);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h[k] = sc * h[k];
     cx<R> m[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) m[e] = lb[e];
@@ -453,18 +545,24 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
     for (int k = 0; k < 2; ++k)
 #pragma unroll
       for (int row = 0; row < 3; ++row)
-        r[3 * k + row] = cmulc(m[0 * 3 + row], h[3 * k + 0]) + cmulc(m[1 * 3 + row], h[3 * k + 1]) +
-                         cmulc(m[2 * 3 + row], h[3 * k + 2]);
+        r[3 * k + row] = row_dotc(m[0 * 3 + row], m[1 * 3 + row], m[2 * 3 + row], h[3 * k + 0],
+                                  h[3 * k + 1], h[3 * k + 2]);
     reconstruct<MU, -SF>(acc, r);
   }
 }
 
 // acc = D_hop^(dag) psi at output site i of parity p_out (wilson_dslash.C:173-517);
-// lu points at this site's 8 staged links, consecutive directions `ds` apart.
+// lu points at this site's 8 staged links, consecutive directions `ds` apart,
+// nbi at its 8 staged neighbour indices, `ns` apart.
 template <int DAG, typename R, class LD>
 __device__ __forceinline__ void hop4(cx<R>* acc, const LD& ld, const cx<R>* lu, int ds,
-                                     const Geom& g, int i, int p_out) {
-  const Nbr nb = neighbours(g, i, p_out);
+                                     const int* nbi, int ns) {
+  Nbr nb;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    nb.fwd[mu] = nbi[mu * ns];
+    nb.bwd[mu] = nbi[(4 + mu) * ns];
+  }
   hop_dir<0, DAG>(acc, ld, lu + 0 * ds, lu + 4 * ds, nb);
   hop_dir<1, DAG>(acc, ld, lu + 1 * ds, lu + 5 * ds, nb);
   hop_dir<2, DAG>(acc, ld, lu + 2 * ds, lu + 6 * ds, nb);
@@ -495,58 +593,102 @@ __device__ __forceinline__ double norm12(const cx<R>* v) {
 }
 
 // ---------------------------------------------------------------------------
+// CG update fused into the last kernel of MatPcDagMatPc (inv_cg.C:340-349):
+// with alpha = scal[0] / scal[1] (|r|^2 and <p, A p> on the device; alpha = 0
+// when <p, A p> == 0 so that the host can report the breakdown on unchanged
+// vectors), r -= alpha A p and x += alpha p at the thread's (site, slice),
+// returning |r|^2 of the site for the partial sums.  A p itself is not stored.
+// ---------------------------------------------------------------------------
+template <class P, class PX>
+struct CgEpi {
+  Fld<P> r;
+  CFld<P> p;
+  Fld<PX> x;
+  const double* scal;
+};
+
+template <class P, class PX>
+__device__ __forceinline__ double cg_epilogue(const CgEpi<P, PX>& cg, const Geom& g, int s,
+                                              int i, const cx<typename P::R>* ap) {
+  typedef typename P::R R;
+  typedef typename PX::R RX;
+  const double rr = cg.scal[0], d = cg.scal[1];
+  const double a = d != 0.0 ? rr / d : 0.0;
+  const size_t sl = (size_t)s * nch<P>() * g.vh, slx = (size_t)s * nch<PX>() * g.vh;
+  const size_t ss = (size_t)s * g.vh;
+  cx<R> r[12], p[12];
+  cx<RX> x[12];
+  ld_spinor<P>(cg.r.d + sl, cg.r.sc + ss, g.vh, i, r);
+  ld_spinor<P>(cg.p.d + sl, cg.p.sc + ss, g.vh, i, p);
+  ld_spinor<PX>(cg.x.d + slx, cg.x.sc + ss, g.vh, i, x);
+#pragma unroll
+  for (int c = 0; c < 12; ++c) {
+    x[c] = cx<RX>{(RX)(a * p[c].re) + x[c].re, (RX)(a * p[c].im) + x[c].im};
+    r[c] = cx<R>{(R)(-a * ap[c].re) + r[c].re, (R)(-a * ap[c].im) + r[c].im};
+  }
+  st_site<PX>(cg.x.d + slx, cg.x.sc + ss, g.vh, i, x);
+  st_site<P>(cg.r.d + sl, cg.r.sc + ss, g.vh, i, r);
+  return norm12(r);
+}
+
+// ---------------------------------------------------------------------------
 // DWF: dwf_dslash = D4 on every slice with 4D parity (s+cb)%2 + 5th-dim hop
 // (dwf_dslash_4.C:107-137, dwf_dslash_5_plus_omp.C:115-270).
 // EPI 0: out = D in.   EPI 1: out = x0 + alpha D in.  NORM: |out|^2 partials.
+// CG (EPI 1): out is not written; the result feeds cg_epilogue instead.
 // ---------------------------------------------------------------------------
-template <class P, int DAG, int EPI, bool NORM>
+template <class P, int DAG, int EPI, bool NORM, bool CG = false, class PX = P>
 __global__ void __launch_bounds__(512)
 k_dwf_dslash(Fld<P> out, CFld<P> in, const typename P::S* __restrict__ U, CFld<P> x0,
              double* __restrict__ partial, Geom g, int cb_in, double two_a5, double nm_two_a5,
-             double alpha) {
+             double alpha, CgEpi<P, PX> cg) {
   typedef typename P::R R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[16];
-  cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw);  // [2 parities][8][ns][9]
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
+  int* lnb = reinterpret_cast<int*>(smem_raw);                   // [2][8][ns]
+  cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw + 2 * 8 * ns * 4);  // [2][8][ns][9]
   const int blk = xcd_block(g.nxcd);
   const int i = blk * ns + il;
   // slice s holds output parity (s + 1 + cb_in) % 2: set s % 2 of the stage
-  stage_links<P>(lu, U, g, blk, ns, 2, (1 + cb_in) & 1);
+  stage_links<P>(lu, lnb, U, g, blk, ns, 2, (1 + cb_in) & 1);
   __syncthreads();
   double nrm = 0.0;
   if (i < g.vh) {
-    const int p_out = (s + 1 + cb_in) & 1;
-    const size_t slice = (size_t)12 * g.vh;
+    const size_t slice = (size_t)nch<P>() * g.vh;
     cx<R> acc[12];
 #pragma unroll
     for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
     hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
-              lu + ((s & 1) * 8 * ns + il) * 9, ns * 9, g, i, p_out);
+              lu + ((s & 1) * 8 * ns + il) * 9, ns * 9, lnb + (s & 1) * 8 * ns + il, ns);
     // fifth dimension: the same site index i in slices s-1 / s+1
     const int sm = s > 0 ? s - 1 : g.ls - 1, spl = s < g.ls - 1 ? s + 1 : 0;
     const R cm = (R)(s > 0 ? two_a5 : nm_two_a5);
     const R cp = (R)(s < g.ls - 1 ? two_a5 : nm_two_a5);
-    const float scm = ld_scale<P>(in.sc + (size_t)sm * g.vh, i);
-    const float scp = ld_scale<P>(in.sc + (size_t)spl * g.vh, i);
     const int lo_m = DAG ? 6 : 0;  // components fed from s-1
     const int lo_p = DAG ? 0 : 6;  // components fed from s+1
+    {
+      cx<R> vm[12], vp[12];
+      ld_spinor<P>(in.d + sm * slice, in.sc + (size_t)sm * g.vh, g.vh, i, vm);
+      ld_spinor<P>(in.d + spl * slice, in.sc + (size_t)spl * g.vh, g.vh, i, vp);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) {
-      acc[lo_m + c] = acc[lo_m + c] + cm * ld<P>(in.d + sm * slice, nullptr, g.vh, lo_m + c, i, scm);
-      acc[lo_p + c] = acc[lo_p + c] + cp * ld<P>(in.d + spl * slice, nullptr, g.vh, lo_p + c, i, scp);
+      for (int c = 0; c < 6; ++c) {
+        acc[lo_m + c] = acc[lo_m + c] + cm * vm[lo_m + c];
+        acc[lo_p + c] = acc[lo_p + c] + cp * vp[lo_p + c];
+      }
     }
     if (EPI == 1) {
-      const float sx = ld_scale<P>(x0.sc + (size_t)s * g.vh, i);
+      cx<R> xv[12];
+      ld_spinor<P>(x0.d + s * slice, x0.sc + (size_t)s * g.vh, g.vh, i, xv);
       const R a = (R)alpha;
 #pragma unroll
-      for (int c = 0; c < 12; ++c)
-        acc[c] = ld<P>(x0.d + s * slice, nullptr, g.vh, c, i, sx) + a * acc[c];
+      for (int c = 0; c < 12; ++c) acc[c] = xv[c] + a * acc[c];
       if (NORM) nrm = norm12(acc);
+      if (CG) nrm = cg_epilogue<P, PX>(cg, g, s, i, acc);
     }
-    st_site<P>(out.d + s * slice, out.sc + (size_t)s * g.vh, g.vh, i, acc);
+    if (!CG) st_site<P>(out.d + s * slice, out.sc + (size_t)s * g.vh, g.vh, i, acc);
   }
-  if (NORM) {
+  if (NORM || CG) {
     const double t = block_sum(nrm, sh);
     if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = t;
   }
@@ -618,12 +760,13 @@ __device__ __forceinline__ void m5inv_columns(cx<R>* col, int sst, int ns, int n
 }
 
 // FL flags of k_mobius
-enum { FL_NORM = 1, FL_PRE_T = 2, FL_POST_T = 4 };
+enum { FL_NORM = 1, FL_PRE_T = 2, FL_POST_T = 4, FL_CG = 8 };
 // FL_NORM   : |out|^2 partials (dag 0 EPI_SCHUR)
 // FL_PRE_T  : dag 0, `in` already holds (b + c D5) applied to the operand, so
 //             the gather is a plain load
 // FL_POST_T : dag 0 EPI_M5INV, write (b + c D5) M5^-1 (...) so that the next
 //             pass can use FL_PRE_T
+// FL_CG     : dag 1 EPI_SCHUR, feed the result to cg_epilogue instead of `out`
 template <typename R>
 __device__ __forceinline__ void apply_t_col(cx<R>* tv, const cx<R>* col_site, int s, int ls,
                                             const MobArgs& a, int dag) {
@@ -646,43 +789,43 @@ template <class P>
 __device__ __forceinline__ void m5_term(cx<typename P::R>* v, CFld<P> psi, const Geom& g, int s,
                                         int i, const MobArgs& a, int dag) {
   typedef typename P::R R;
-  const size_t slice = (size_t)12 * g.vh;
+  const size_t slice = (size_t)nch<P>() * g.vh;
   const int ls = g.ls;
   const int sm = s > 0 ? s - 1 : ls - 1, sp = s < ls - 1 ? s + 1 : 0;
-  const float s0 = ld_scale<P>(psi.sc + (size_t)s * g.vh, i);
-  const float fm = ld_scale<P>(psi.sc + (size_t)sm * g.vh, i);
-  const float fp = ld_scale<P>(psi.sc + (size_t)sp * g.vh, i);
   const R km = (R)(s > 0 ? a.K : a.mK), kp = (R)(s < ls - 1 ? a.K : a.mK);
+  cx<R> vm[12], vp[12];
+  ld_spinor<P>(psi.d + s * slice, psi.sc + (size_t)s * g.vh, g.vh, i, v);
+  ld_spinor<P>(psi.d + sm * slice, psi.sc + (size_t)sm * g.vh, g.vh, i, vm);
+  ld_spinor<P>(psi.d + sp * slice, psi.sc + (size_t)sp * g.vh, g.vh, i, vp);
 #pragma unroll
   for (int c = 0; c < 12; ++c) {
     const bool from_minus = (c < 6) == (dag == 0);
-    const cx<R> d5 = from_minus ? km * ld<P>(psi.d + sm * slice, nullptr, g.vh, c, i, fm)
-                                : kp * ld<P>(psi.d + sp * slice, nullptr, g.vh, c, i, fp);
-    v[c] = ld<P>(psi.d + s * slice, nullptr, g.vh, c, i, s0) + d5;
+    v[c] = v[c] + (from_minus ? km * vm[c] : kp * vp[c]);
   }
 }
 
-template <class P, int DAG, int EPI, int FL>
+template <class P, int DAG, int EPI, int FL, class PX = P>
 __global__ void __launch_bounds__(512)
 k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ U,
-         double* __restrict__ partial, Geom g, int p_out, MobArgs a) {
+         double* __restrict__ partial, Geom g, int p_out, MobArgs a, CgEpi<P, PX> cg) {
   typedef typename P::R R;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[16];
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
-  cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw);  // staged links [8][ns][9]
-  cx<R>* col = lu + 8 * ns * 9;                     // s-columns [ns][sst]
+  int* lnb = reinterpret_cast<int*>(smem_raw);                   // neighbours [8][ns]
+  cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw + 8 * ns * 4);  // staged links [8][ns][9]
+  cx<R>* col = lu + 8 * ns * 9;                                  // s-columns [ns][sst]
   const int sst = g.ls * 12 + 1;                    // padded site stride in LDS
   const int blk = xcd_block(g.nxcd);
   const int i = blk * ns + il;
   const bool act = i < g.vh;
   const int nvalid = min(ns, g.vh - blk * ns);
-  const size_t slice = (size_t)12 * g.vh;
+  const size_t slice = (size_t)nch<P>() * g.vh;
   const int ls = g.ls;
-  typename P::S* od = out.d + s * slice;
+  typename P::C* od = out.d + s * slice;
   float* osc = out.sc + (size_t)s * g.vh;
 
-  stage_links<P>(lu, U, g, blk, ns, 1, p_out);
+  stage_links<P>(lu, lnb, U, g, blk, ns, 1, p_out);
   __syncthreads();
   cx<R> acc[12];
 #pragma unroll
@@ -701,10 +844,10 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
       ld.b = (R)a.b;
       ld.cm = (R)(s > 0 ? a.c : a.mc);
       ld.cp = (R)(s < ls - 1 ? a.c : a.mc);
-      hop4<0>(acc, ld, lu + il * 9, ns * 9, g, i, p_out);
+      hop4<0>(acc, ld, lu + il * 9, ns * 9, lnb + il, ns);
     } else {
       hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
-                lu + il * 9, ns * 9, g, i, p_out);
+                lu + il * 9, ns * 9, lnb + il, ns);
     }
   }
 
@@ -744,12 +887,20 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
       return;
     }
     if (EPI == EPI_SCHUR) {
+      double nrm = 0.0;
       if (act) {
         cx<R> v[12];
         m5_term<P>(v, psi, g, s, i, a, 1);
 #pragma unroll
         for (int c = 0; c < 12; ++c) v[c] = v[c] + tv[c];
-        st_site<P>(od, osc, g.vh, i, v);
+        if (FL & FL_CG)
+          nrm = cg_epilogue<P, PX>(cg, g, s, i, v);
+        else
+          st_site<P>(od, osc, g.vh, i, v);
+      }
+      if (FL & FL_CG) {
+        const double t = block_sum(nrm, sh);
+        if (threadIdx.x == 0 && threadIdx.y == 0) partial[blockIdx.x] = t;
       }
       return;
     }
@@ -792,12 +943,13 @@ k_m5inv(Fld<P> out, CFld<P> in, Geom g, M5Side k) {
   const int i = blk * ns + il;
   const bool act = i < g.vh;
   const int nvalid = min(ns, g.vh - blk * ns);
-  const size_t slice = (size_t)12 * g.vh;
+  const size_t slice = (size_t)nch<P>() * g.vh;
   cx<R>* mycol = col + il * sst;
   if (act) {
-    const float sc = ld_scale<P>(in.sc + (size_t)s * g.vh, i);
+    cx<R> v[12];
+    ld_spinor<P>(in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh, i, v);
 #pragma unroll
-    for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = ld<P>(in.d + s * slice, nullptr, g.vh, c, i, sc);
+    for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = v[c];
   }
   __syncthreads();
   m5inv_columns<DAG>(col, sst, ns, nvalid, g.ls, k);
@@ -817,17 +969,14 @@ template <class P>
 __device__ __forceinline__ void ld_site(cx<typename P::R>* v, CFld<P> f, const Geom& g, size_t k) {
   // k = s*vh + i
   const size_t s = k / g.vh, i = k - s * g.vh;
-  const float sc = ld_scale<P>(f.sc, (int)k);
-  const typename P::S* d = f.d + s * 12 * (size_t)g.vh + i;
-#pragma unroll
-  for (int c = 0; c < 12; ++c) v[c] = Pack<P>::unpack(d[(size_t)c * g.vh], sc);
+  ld_spinor<P>(f.d + s * nch<P>() * (size_t)g.vh, f.sc + s * (size_t)g.vh, g.vh, (int)i, v);
 }
 
 template <class P>
 __device__ __forceinline__ void st_site_k(Fld<P> f, const Geom& g, size_t k,
                                           const cx<typename P::R>* v) {
   const size_t s = k / g.vh, i = k - s * g.vh;
-  st_site<P>(f.d + s * 12 * (size_t)g.vh, f.sc + s * (size_t)g.vh, g.vh, (int)i, v);
+  st_site<P>(f.d + s * nch<P>() * (size_t)g.vh, f.sc + s * (size_t)g.vh, g.vh, (int)i, v);
 }
 
 #define DWFK_SITE_LOOP(k, g) \
