@@ -89,10 +89,21 @@ struct dwfgpu_s {
   int grid_hop = 0;
   int ns = 16;  // sites per work-group of the hopping kernels (x); y = all slices
   int max_lds = 65536;
+  // launch shape of the many-RHS operator chosen by the autotuner, per
+  // sloppy precision and RHS count (0: not tuned yet)
+  int mr_ns[NPREC] = {0, 0, 0, 0}, mr_nr[NPREC] = {0, 0, 0, 0}, mr_nrhs[NPREC] = {0, 0, 0, 0};
+  int mr_last_ns = 0, mr_last_nr = 0;
 };
 
 struct dwfgpu_vec_s {
   dwfgpu_s* h = nullptr;
+  DVec v;
+};
+
+// nrhs FP64 half vectors in one allocation, RHS j at chunk offset j * n/2
+struct dwfgpu_mvec_s {
+  dwfgpu_s* h = nullptr;
+  int nrhs = 0;
   DVec v;
 };
 
@@ -843,6 +854,436 @@ const DVec& api_work(dwfgpu_s* h, int k) { return h->f[h->prec].work[k]; }
 
 }  // namespace
 
+
+// ===========================================================================
+// Many right-hand sides (DWF): batched operator and the batched
+// reliable-update CG with device-side retirement and a graph-captured step
+// ===========================================================================
+namespace {
+
+// a slice of an nrhs-vector allocation of policy P
+template <class P>
+DVec mslice(const dwfgpu_s* h, const DVec& v, int j) {
+  const size_t vs = h->n / 2 / P::kCPC;  // chunks per vector
+  DVec r;
+  r.d = (char*)v.d + (size_t)j * vs * sizeof(typename P::C);
+  r.sc = v.sc ? v.sc + (size_t)j * h->g.vh * h->ls : nullptr;
+  return r;
+}
+
+template <class P>
+int alloc_mvec(dwfgpu_s* h, int nrhs, DVec* v) {
+  TRY(alloc(h, &v->d, (size_t)nrhs * (h->n / 2) * sizeof(typename P::S)));
+  if (P::kScaled)
+    TRY(alloc(h, (void**)&v->sc, (size_t)nrhs * h->g.vh * h->ls * sizeof(float)));
+  return DWFGPU_OK;
+}
+
+struct MrShape {
+  int ns, nr;
+};
+
+template <class PS>
+struct MrState {
+  int nrhs = 0;
+  DVec te, t1, rs, ps, xs;  // xs in P32
+  double* partial = nullptr;
+  double* scal = nullptr;
+  double* mass = nullptr;
+  int* ibuf = nullptr;  // active | flag | iters
+  int gx = 0;
+  ~MrState() {
+    free_vec(&te);
+    free_vec(&t1);
+    free_vec(&rs);
+    free_vec(&ps);
+    free_vec(&xs);
+    void* p[] = {partial, scal, mass, ibuf};
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+  }
+  int* active() const { return ibuf; }
+  int* flag() const { return ibuf + nrhs; }
+  int* iters() const { return ibuf + 2 * nrhs; }
+};
+
+size_t mr_lds(const dwfgpu_s* h, const MrShape& t, size_t creal) {
+  return (size_t)2 * 8 * t.ns * (9 * 2 * creal + 4) + (size_t)t.ns * h->ls * t.nr * 8;
+}
+
+// kind 0: out = D p (cb 1)          kind 1: out = x0 + alpha D in, |out|^2
+// kind 2: out = D^dag in (cb 1)      kind 3: x0 + alpha D^dag in -> CG epilogue
+// kind 4: out = x0 + alpha D^dag in (plain batched MatPcDagMatPc)
+template <class P>
+int mr_launch(dwfgpu_s* h, const MrShape& t, int nrhs, dwfk::MrArgs<P, P32> a, int kind) {
+  const dim3 grid((h->g.vh + t.ns - 1) / t.ns, (nrhs + t.nr - 1) / t.nr);
+  const dim3 block(t.ns, h->ls, t.nr);
+  const size_t lds = mr_lds(h, t, sizeof(typename P::R));
+  a.cb_in = (kind == 0 || kind == 2) ? 1 : 0;
+  switch (kind) {
+    case 0:
+      hipLaunchKernelGGL((dwfk::k_dwf_mrhs<P, 0, 0, false, false, P32>), grid, block, lds, h->st, a);
+      break;
+    case 1:
+      hipLaunchKernelGGL((dwfk::k_dwf_mrhs<P, 0, 1, true, false, P32>), grid, block, lds, h->st, a);
+      break;
+    case 2:
+      hipLaunchKernelGGL((dwfk::k_dwf_mrhs<P, 1, 0, false, false, P32>), grid, block, lds, h->st, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((dwfk::k_dwf_mrhs<P, 1, 1, false, true, P32>), grid, block, lds, h->st, a);
+      break;
+    default:
+      hipLaunchKernelGGL((dwfk::k_dwf_mrhs<P, 1, 1, false, false, P32>), grid, block, lds, h->st, a);
+      break;
+  }
+  HIPCHK(hipGetLastError());
+  return DWFGPU_OK;
+}
+
+// batched MatPcDagMatPc (cg: the CG step of every active RHS instead of `out`)
+template <class P>
+int mr_mdagm(dwfgpu_s* h, const MrShape& t, int nrhs, const DVec& out, const DVec& in,
+             const DVec& te, const DVec& t1, const double* mass, const int* active,
+             double* partial, double* scal, const DVec* r, const DVec* x, bool cg) {
+  const double kap = dwf_kappa(h);
+  dwfk::MrArgs<P, P32> a{};
+  a.U = (const typename P::S*)h->f[pidx<P>()].U;
+  a.partial = partial;
+  a.g = h->g;
+  a.two_a5 = 2.0 * h->a5inv;
+  a.mass = mass;
+  a.active = active;
+  a.vs = h->n / 2 / P::kCPC;
+  a.nrhs = nrhs;
+  a.scal = scal;
+  const int gx = (h->g.vh + t.ns - 1) / t.ns;
+  // te = D in
+  a.out = fld<P>(te);
+  a.in = cfld<P>(in);
+  TRY(mr_launch<P>(h, t, nrhs, a, 0));
+  // t1 = in - kappa^2 D te, |t1|^2 -> scal[D]
+  a.out = fld<P>(t1);
+  a.in = cfld<P>(te);
+  a.x0 = cfld<P>(in);
+  a.alpha = -kap * kap;
+  TRY(mr_launch<P>(h, t, nrhs, a, 1));
+  if (scal) {
+    hipLaunchKernelGGL(dwfk::k_reduce_mrhs, dim3(nrhs), dim3(1024), 0, h->st, partial, gx, scal,
+                       (int)dwfk::MR_D, active);
+    HIPCHK(hipGetLastError());
+  }
+  // te = D^dag t1
+  a.out = fld<P>(te);
+  a.in = cfld<P>(t1);
+  a.x0 = dwfk::CFld<P>{};
+  a.alpha = 0.0;
+  TRY(mr_launch<P>(h, t, nrhs, a, 2));
+  // t1 - kappa^2 D^dag te: into out, or into the CG step
+  a.in = cfld<P>(te);
+  a.x0 = cfld<P>(t1);
+  a.alpha = -kap * kap;
+  if (cg) {
+    a.r = fld<P>(*r);
+    a.p = cfld<P>(in);
+    a.x = fld<P32>(*x);
+    TRY(mr_launch<P>(h, t, nrhs, a, 3));
+    hipLaunchKernelGGL(dwfk::k_reduce_mrhs, dim3(nrhs), dim3(1024), 0, h->st, partial, gx, scal,
+                       (int)dwfk::MR_RRNEW, active);
+    HIPCHK(hipGetLastError());
+  } else {
+    a.out = fld<P>(out);
+    TRY(mr_launch<P>(h, t, nrhs, a, 4));
+  }
+  return DWFGPU_OK;
+}
+
+// launch shapes the autotuner tries: ns sites x Ls slices x nr RHS threads
+std::vector<MrShape> mr_candidates(const dwfgpu_s* h, int nrhs, size_t creal) {
+  std::vector<MrShape> c;
+  for (int ns : {8, 16, 32})
+    for (int nr : {1, 2, 4}) {
+      if (nr > nrhs || ns * h->ls * nr > 512 || ns * h->ls * nr < 64) continue;
+      const MrShape t{ns, nr};
+      if (mr_lds(h, t, creal) > (size_t)h->max_lds) continue;
+      c.push_back(t);
+    }
+  if (c.empty()) c.push_back(MrShape{h->ns, 1});
+  return c;
+}
+
+// Project-written autotuner: time one batched MatPcDagMatPc (all RHS active)
+// for every candidate shape, twice, and keep the fastest; cached per handle,
+// sloppy precision and RHS count.
+template <class P>
+int mr_tune(dwfgpu_s* h, int nrhs, MrState<P>& m, const int* all_active, MrShape* best) {
+  const int pi = pidx<P>();
+  if (h->mr_ns[pi] && h->mr_nrhs[pi] == nrhs) {
+    *best = MrShape{h->mr_ns[pi], h->mr_nr[pi]};
+    return DWFGPU_OK;
+  }
+  const char* env = getenv("DWFGPU_MRHS_SHAPE");  // "ns,nr" pins the shape
+  if (env) {
+    int ns = 0, nr = 0;
+    if (sscanf(env, "%d,%d", &ns, &nr) == 2 && ns > 0 && nr > 0 && ns * h->ls * nr <= 512) {
+      *best = MrShape{ns, nr};
+      h->mr_ns[pi] = ns;
+      h->mr_nr[pi] = nr;
+      h->mr_nrhs[pi] = nrhs;
+      return DWFGPU_OK;
+    }
+  }
+  const auto cand = mr_candidates(h, nrhs, sizeof(typename P::R));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    set_err("mr_tune: hipEventCreate failed");
+    return DWFGPU_HIP_ERROR;
+  }
+  float best_ms = 1e30f;
+  int st = DWFGPU_OK;
+  for (const MrShape& t : cand) {
+    float tot = 0;
+    for (int rep = 0; rep < 3 && !st; ++rep) {
+      if (hipEventRecord(e0, h->st) != hipSuccess) st = DWFGPU_HIP_ERROR;
+      if (!st)
+        st = mr_mdagm<P>(h, t, nrhs, m.rs, m.ps, m.te, m.t1, m.mass, all_active, m.partial, nullptr,
+                         nullptr, nullptr, false);
+      if (!st && (hipEventRecord(e1, h->st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess))
+        st = DWFGPU_HIP_ERROR;
+      float ms = 0;
+      if (!st && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) st = DWFGPU_HIP_ERROR;
+      if (rep > 0) tot += ms;  // the first run only warms up
+    }
+    if (st) break;
+    if (tot < best_ms) {
+      best_ms = tot;
+      *best = t;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) {
+    set_err("mr_tune: timing failed");
+    return st;
+  }
+  h->mr_ns[pi] = best->ns;
+  h->mr_nr[pi] = best->nr;
+  h->mr_nrhs[pi] = nrhs;
+  return DWFGPU_OK;
+}
+
+template <class PS>
+int invcg_mrhs(dwfgpu_s* h, const dwfgpu_cg_arg* arg, int nrhs, const double* masses,
+               double delta, int flags, const DVec& sol, const DVec& src, double* true_res,
+               int* iters, int* nrel, int* status) {
+  typedef Ops<P64> H;
+  typedef Ops<PS> L;
+  const Geom& g = h->g;
+  TRY(ensure_work<P64>(h, 2));
+  TRY(ensure_fields<PS>(h));
+  const DVec& r = h->f[DWFGPU_F64].work[0];
+  const DVec& ax = h->f[DWFGPU_F64].work[1];
+  MrState<PS> m;
+  m.nrhs = nrhs;
+  m.gx = (g.vh + 8 - 1) / 8;  // partials sized for the smallest candidate ns
+  TRY(alloc_mvec<PS>(h, nrhs, &m.te));
+  TRY(alloc_mvec<PS>(h, nrhs, &m.t1));
+  TRY(alloc_mvec<PS>(h, nrhs, &m.rs));
+  TRY(alloc_mvec<PS>(h, nrhs, &m.ps));
+  TRY(alloc_mvec<P32>(h, nrhs, &m.xs));
+  TRY(alloc(h, (void**)&m.partial, (size_t)nrhs * m.gx * sizeof(double)));
+  TRY(alloc(h, (void**)&m.scal, (size_t)nrhs * dwfk::MR_NSCAL * sizeof(double)));
+  TRY(alloc(h, (void**)&m.mass, (size_t)nrhs * sizeof(double)));
+  TRY(alloc(h, (void**)&m.ibuf, (size_t)3 * nrhs * sizeof(int)));
+  HIPCHK(hipMemcpyAsync(m.mass, masses, nrhs * sizeof(double), hipMemcpyHostToDevice, h->st));
+
+  std::vector<double> sc((size_t)nrhs * dwfk::MR_NSCAL, 0.0), src_nsq(nrhs, 0.0);
+  std::vector<int> act(nrhs, 0), flg(nrhs, 0), its(nrhs, 0), done(nrhs, 0);
+  const int max_itr = arg->max_num_iter - 1;
+  for (int j = 0; j < nrhs; ++j) {
+    nrel[j] = 0;
+    status[j] = DWFGPU_OK;
+  }
+
+  // autotune with every RHS marked active (operands: the zeroed sloppy fields)
+  {
+    std::vector<int> ones(nrhs, 1);
+    HIPCHK(hipMemcpyAsync(m.ibuf, ones.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, h->st));
+  }
+  MrShape t{};
+  TRY(mr_tune<PS>(h, nrhs, m, m.active(), &t));
+  h->mr_last_ns = t.ns;
+  h->mr_last_nr = t.nr;
+
+  // initial residuals in FP64, one RHS at a time
+  for (int j = 0; j < nrhs; ++j) {
+    const DVec xj = mslice<P64>(h, sol, j), bj = mslice<P64>(h, src, j);
+    TRY(H::norm(h, bj, &src_nsq[j]));
+    const double stp = src_nsq[j] * arg->stop_rsd * arg->stop_rsd;
+    TRY(H::mdagm(h, masses[j], ax, xj, nullptr));
+    TRY(H::axpy_to(h, r, ax, bj, -1.0, true));
+    double rr;
+    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
+    double* sj = &sc[(size_t)j * dwfk::MR_NSCAL];
+    sj[dwfk::MR_RR] = rr;
+    sj[dwfk::MR_RRUPD] = rr;
+    sj[dwfk::MR_STOP] = stp;
+    sj[dwfk::MR_DELTA2] = delta * delta;
+    sj[dwfk::MR_MAXIT] = max_itr;
+    if (rr <= stp || max_itr <= 0) {
+      done[j] = 1;
+      true_res[j] = std::sqrt(rr / src_nsq[j]);
+      if (rr > stp) status[j] = DWFGPU_MAXITER;
+      continue;
+    }
+    const DVec rsj = mslice<PS>(h, m.rs, j), psj = mslice<PS>(h, m.ps, j);
+    hipLaunchKernelGGL((dwfk::k_convert<PS, P64>), blas_grid(), blas_block(), 0, h->st, fld<PS>(rsj),
+                       cfld<P64>(r), g);
+    HIPCHK(hipGetLastError());
+    TRY(L::copy(h, psj, rsj));
+    act[j] = 1;
+  }
+  HIPCHK(hipMemsetAsync(m.xs.d, 0, (size_t)nrhs * (h->n / 2) * sizeof(cx<float>), h->st));
+  HIPCHK(hipMemcpyAsync(m.scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(m.ibuf, act.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(m.flag(), flg.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipMemcpyAsync(m.iters(), its.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, h->st));
+
+  const size_t vs = h->n / 2 / PS::kCPC;
+  auto step = [&]() -> int {
+    TRY(mr_mdagm<PS>(h, t, nrhs, DVec(), m.ps, m.te, m.t1, m.mass, m.active(), m.partial, m.scal,
+                     &m.rs, &m.xs, true));
+    hipLaunchKernelGGL(dwfk::k_cg_scalars_mrhs, dim3(1), dim3(256), 0, h->st, m.scal, m.active(),
+                       m.flag(), m.iters(), nrhs);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL((dwfk::k_xpby_mrhs<PS>), dim3(dwfk::BLAS_BLOCKS / 4, nrhs), blas_block(), 0,
+                       h->st, fld<PS>(m.ps), cfld<PS>(m.rs), m.scal, m.active(), vs, g);
+    HIPCHK(hipGetLastError());
+    return DWFGPU_OK;
+  };
+
+  // K CG steps per host round, captured once into a graph (no memset or
+  // copy inside it) unless DWFGPU_MRHS_EAGER asks for plain launches
+  const int K = 4;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int st = DWFGPU_OK;
+  if (!(flags & DWFGPU_MRHS_EAGER)) {
+    HIPCHK(hipStreamSynchronize(h->st));
+    HIPCHK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < K && !st; ++k) st = step();
+    hipGraph_t gcap = nullptr;
+    const hipError_t ec = hipStreamEndCapture(h->st, &gcap);
+    if (st || ec != hipSuccess) {
+      if (gcap) (void)hipGraphDestroy(gcap);
+      set_err("invcg_mrhs: graph capture failed");
+      return st ? st : DWFGPU_HIP_ERROR;
+    }
+    graph = gcap;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      set_err("invcg_mrhs: graph instantiate failed");
+      return DWFGPU_HIP_ERROR;
+    }
+  }
+  auto cleanup = [&]() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  };
+  // every RHS either retires or gains at least one iteration per round, and
+  // its iterations are capped at max_itr, so the loop ends
+  while (!st) {
+    int nact = 0;
+    for (int j = 0; j < nrhs; ++j) nact += act[j];
+    if (nact == 0) break;
+    if (exec) {
+      if (hipGraphLaunch(exec, h->st) != hipSuccess) st = DWFGPU_HIP_ERROR;
+    } else {
+      for (int k = 0; k < K && !st; ++k) st = step();
+    }
+    if (st) break;
+    std::vector<int> ib(3 * nrhs);
+    if (hipMemcpyAsync(ib.data(), m.ibuf, ib.size() * sizeof(int), hipMemcpyDeviceToHost, h->st) !=
+            hipSuccess ||
+        hipMemcpyAsync(sc.data(), m.scal, sc.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       h->st) != hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess) {
+      st = DWFGPU_HIP_ERROR;
+      break;
+    }
+    bool changed = false;
+    for (int j = 0; j < nrhs && !st; ++j) {
+      act[j] = ib[j];
+      flg[j] = ib[nrhs + j];
+      its[j] = ib[2 * nrhs + j];
+      if (flg[j] == 0) continue;
+      changed = true;
+      double* sj = &sc[(size_t)j * dwfk::MR_NSCAL];
+      const DVec xj = mslice<P64>(h, sol, j), bj = mslice<P64>(h, src, j);
+      const DVec rsj = mslice<PS>(h, m.rs, j), psj = mslice<PS>(h, m.ps, j);
+      const DVec xsj = mslice<P32>(h, m.xs, j);
+      if (flg[j] == 2) {  // <p, A p> == 0: CPS's breakdown
+        status[j] = DWFGPU_BREAKDOWN;
+        done[j] = 1;
+        act[j] = 0;
+        flg[j] = 0;
+        continue;
+      }
+      // reliable update: x += xs, r = b - A x in FP64
+      hipLaunchKernelGGL((dwfk::k_accumulate<P32>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P64>(xj), fld<P32>(xsj), g);
+      if (hipGetLastError() != hipSuccess) st = DWFGPU_HIP_ERROR;
+      if (!st) st = H::mdagm(h, masses[j], ax, xj, nullptr);
+      if (!st) st = H::axpy_to(h, r, ax, bj, -1.0, true);
+      double rr = 0;
+      if (!st) st = fetch_sum(h, dwfk::BLAS_BLOCKS, &rr);
+      if (st) break;
+      ++nrel[j];
+      if (rr <= sj[dwfk::MR_STOP] || its[j] >= max_itr || flg[j] == 3) {
+        done[j] = 1;
+        act[j] = 0;
+        true_res[j] = std::sqrt(rr / src_nsq[j]);
+        if (rr > sj[dwfk::MR_STOP]) status[j] = DWFGPU_MAXITER;
+        flg[j] = 0;
+        continue;
+      }
+      // continue: r_s = r, p_s = r_s + beta' p_s with beta' = rr / rr_prev
+      hipLaunchKernelGGL((dwfk::k_convert<PS, P64>), blas_grid(), blas_block(), 0, h->st,
+                         fld<PS>(rsj), cfld<P64>(r), g);
+      if (hipGetLastError() != hipSuccess) st = DWFGPU_HIP_ERROR;
+      const double beta = sj[dwfk::MR_BETA] * rr / sj[dwfk::MR_RR];
+      if (!st) st = L::axpby(h, psj, rsj, 1.0, beta);
+      sj[dwfk::MR_RR] = rr;
+      sj[dwfk::MR_RRUPD] = rr;
+      act[j] = 1;
+      flg[j] = 0;
+    }
+    if (st || !changed) continue;
+    for (int j = 0; j < nrhs; ++j) {
+      ib[j] = act[j];
+      ib[nrhs + j] = flg[j];
+    }
+    if (hipMemcpyAsync(m.scal, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice,
+                       h->st) != hipSuccess ||
+        hipMemcpyAsync(m.ibuf, ib.data(), 2 * nrhs * sizeof(int), hipMemcpyHostToDevice, h->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(h->st) != hipSuccess)
+      st = DWFGPU_HIP_ERROR;
+  }
+  cleanup();
+  if (st) {
+    if (g_err.empty()) set_err("invcg_mrhs: HIP failure");
+    return st;
+  }
+  for (int j = 0; j < nrhs; ++j) iters[j] = its[j] + 1;
+  HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
+}
+
+}  // namespace
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1226,6 +1667,135 @@ int dwfgpu_time_matpcdagmatpc(dwfgpu_t h, int precision, double mass, dwfgpu_vec
     case DWFGPU_I16: return run(PI16());
     default: return DWFGPU_BAD_ARG;
   }
+}
+
+int dwfgpu_mvec_create(dwfgpu_t h, int nrhs, dwfgpu_mvec_t* out) {
+  if (!h || !out || nrhs < 1) return DWFGPU_BAD_ARG;
+  if (h->prec != DWFGPU_F64) {
+    set_err("dwfgpu_mvec_create: needs an FP64 handle");
+    return DWFGPU_BAD_ARG;
+  }
+  dwfgpu_mvec_s* v = new dwfgpu_mvec_s;
+  v->h = h;
+  v->nrhs = nrhs;
+  int st = alloc_mvec<P64>(h, nrhs, &v->v);
+  if (!st && hipStreamSynchronize(h->st) != hipSuccess) st = DWFGPU_HIP_ERROR;
+  if (st) {
+    free_vec(&v->v);
+    delete v;
+    return st;
+  }
+  *out = v;
+  return DWFGPU_OK;
+}
+
+int dwfgpu_mvec_destroy(dwfgpu_mvec_t v) {
+  if (!v) return DWFGPU_BAD_ARG;
+  (void)hipStreamSynchronize(v->h->st);
+  free_vec(&v->v);
+  delete v;
+  return DWFGPU_OK;
+}
+
+int dwfgpu_mvec_upload(dwfgpu_mvec_t v, int j, const double* host) {
+  if (!v || !host || j < 0 || j >= v->nrhs) return DWFGPU_BAD_ARG;
+  TRY(Ops<P64>::to_device(v->h, mslice<P64>(v->h, v->v, j), host));
+  HIPCHK(hipStreamSynchronize(v->h->st));
+  return DWFGPU_OK;
+}
+
+int dwfgpu_mvec_download(dwfgpu_mvec_t v, int j, double* host) {
+  if (!v || !host || j < 0 || j >= v->nrhs) return DWFGPU_BAD_ARG;
+  return Ops<P64>::to_host(v->h, host, mslice<P64>(v->h, v->v, j));
+}
+
+int dwfgpu_matpcdagmatpc_mrhs_dev(dwfgpu_t h, int precision, const double* masses,
+                                  dwfgpu_mvec_t out, dwfgpu_mvec_t in) {
+  if (!h || !masses || !out || !in || out->h != h || in->h != h || out->nrhs != in->nrhs)
+    return DWFGPU_BAD_ARG;
+  if (h->action != DWFGPU_DWF) {
+    set_err("dwfgpu_matpcdagmatpc_mrhs_dev: the batched operator is DWF only");
+    return DWFGPU_BAD_ARG;
+  }
+  const int nrhs = in->nrhs;
+  auto run = [&](auto tag) -> int {
+    typedef decltype(tag) P;
+    TRY(ensure_fields<P>(h));
+    MrState<P> m;
+    m.nrhs = nrhs;
+    m.gx = (h->g.vh + 7) / 8;
+    TRY(alloc_mvec<P>(h, nrhs, &m.te));
+    TRY(alloc_mvec<P>(h, nrhs, &m.t1));
+    TRY(alloc_mvec<P>(h, nrhs, &m.rs));
+    TRY(alloc_mvec<P>(h, nrhs, &m.ps));
+    TRY(alloc(h, (void**)&m.partial, (size_t)nrhs * m.gx * sizeof(double)));
+    TRY(alloc(h, (void**)&m.mass, (size_t)nrhs * sizeof(double)));
+    TRY(alloc(h, (void**)&m.ibuf, (size_t)3 * nrhs * sizeof(int)));
+    std::vector<int> ones(nrhs, 1);
+    HIPCHK(hipMemcpyAsync(m.ibuf, ones.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(m.mass, masses, nrhs * sizeof(double), hipMemcpyHostToDevice, h->st));
+    for (int j = 0; j < nrhs; ++j)
+      hipLaunchKernelGGL((dwfk::k_convert<P, P64>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P>(mslice<P>(h, m.ps, j)), cfld<P64>(mslice<P64>(h, in->v, j)), h->g);
+    HIPCHK(hipGetLastError());
+    MrShape t{};
+    TRY(mr_tune<P>(h, nrhs, m, m.active(), &t));
+    h->mr_last_ns = t.ns;
+    h->mr_last_nr = t.nr;
+    TRY(mr_mdagm<P>(h, t, nrhs, m.rs, m.ps, m.te, m.t1, m.mass, m.active(), m.partial, nullptr,
+                    nullptr, nullptr, false));
+    for (int j = 0; j < nrhs; ++j)
+      hipLaunchKernelGGL((dwfk::k_convert<P64, P>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P64>(mslice<P64>(h, out->v, j)), cfld<P>(mslice<P>(h, m.rs, j)), h->g);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    return DWFGPU_OK;
+  };
+  switch (precision) {
+    case DWFGPU_F64: return run(P64());
+    case DWFGPU_F32: return run(P32());
+    case DWFGPU_BF16: return run(PBF16());
+    case DWFGPU_I16: return run(PI16());
+    default: return DWFGPU_BAD_ARG;
+  }
+}
+
+int dwfgpu_invcg_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, double delta,
+                          int flags, const double* masses, dwfgpu_mvec_t sol, dwfgpu_mvec_t src,
+                          double* true_res, int* iters, int* reliable_updates, int* status) {
+  if (!h || !arg || !masses || !sol || !src || !true_res || !iters || !reliable_updates ||
+      !status || sol->h != h || src->h != h || sol->nrhs != src->nrhs)
+    return DWFGPU_BAD_ARG;
+  if (h->action != DWFGPU_DWF) {
+    set_err("dwfgpu_invcg_mrhs_dev: the batched solver is DWF only");
+    return DWFGPU_BAD_ARG;
+  }
+  if (arg->max_num_iter < 1 || !(delta > 0.0 && delta < 1.0)) {
+    set_err("dwfgpu_invcg_mrhs_dev: need max_num_iter >= 1 and 0 < delta < 1");
+    return DWFGPU_BAD_ARG;
+  }
+  const int nrhs = sol->nrhs;
+  switch (sloppy) {
+    case DWFGPU_F32:
+      return invcg_mrhs<P32>(h, arg, nrhs, masses, delta, flags, sol->v, src->v, true_res, iters,
+                             reliable_updates, status);
+    case DWFGPU_BF16:
+      return invcg_mrhs<PBF16>(h, arg, nrhs, masses, delta, flags, sol->v, src->v, true_res,
+                               iters, reliable_updates, status);
+    case DWFGPU_I16:
+      return invcg_mrhs<PI16>(h, arg, nrhs, masses, delta, flags, sol->v, src->v, true_res, iters,
+                              reliable_updates, status);
+    default:
+      set_err("dwfgpu_invcg_mrhs_dev: sloppy precision must be F32, BF16 or I16");
+      return DWFGPU_BAD_ARG;
+  }
+}
+
+int dwfgpu_mrhs_shape(dwfgpu_t h, int* ns, int* nr) {
+  if (!h || !ns || !nr) return DWFGPU_BAD_ARG;
+  *ns = h->mr_last_ns;
+  *nr = h->mr_last_nr;
+  return DWFGPU_OK;
 }
 
 }  // extern "C"

@@ -54,11 +54,14 @@ template <typename R>
 __device__ __forceinline__ cx<R> cmulc(cx<R> a, cx<R> b) {
   return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
 }
-// U h and U^dag h for one colour vector h (3 complex), as chains of fused
-// multiply-adds (4 per complex product, no separate sums)
+// One row of U h and of U^dag h (3 complex products summed).  FP32 (VALU-
+// bound kernels): chains of fused multiply-adds, the fewest instructions.
+// FP64 (HBM-bound, two waves per SIMD): three independent products then a
+// sum, which leaves more independent work to hide FP64 latency.
 template <typename R>
 __device__ __forceinline__ cx<R> row_dot(cx<R> a0, cx<R> a1, cx<R> a2, cx<R> b0, cx<R> b1,
                                          cx<R> b2) {
+  if constexpr (sizeof(R) == 8) return cmul(a0, b0) + cmul(a1, b1) + cmul(a2, b2);
   R re = a0.re * b0.re;
   re = fma(-a0.im, b0.im, re);
   re = fma(a1.re, b1.re, re);
@@ -77,6 +80,7 @@ __device__ __forceinline__ cx<R> row_dot(cx<R> a0, cx<R> a1, cx<R> a2, cx<R> b0,
 template <typename R>
 __device__ __forceinline__ cx<R> row_dotc(cx<R> a0, cx<R> a1, cx<R> a2, cx<R> b0, cx<R> b1,
                                           cx<R> b2) {
+  if constexpr (sizeof(R) == 8) return cmulc(a0, b0) + cmulc(a1, b1) + cmulc(a2, b2);
   R re = a0.re * b0.re;
   re = fma(a0.im, b0.im, re);
   re = fma(a1.re, b1.re, re);
@@ -470,8 +474,8 @@ __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
                                             int blk, int ns, int nsets, int p_first) {
   typedef typename P::R R;
   constexpr int NE = P::kLinkElems;
-  const int tid = threadIdx.x + blockDim.x * threadIdx.y;
-  const int nthr = blockDim.x * blockDim.y;
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int nthr = blockDim.x * blockDim.y * blockDim.z;
   for (int w = tid; w < nsets * 8 * ns; w += nthr) {
     const int il = w % ns, d = (w / ns) % 8, set = w / (8 * ns);
     const int i = blk * ns + il;
@@ -1185,6 +1189,212 @@ k_compress_links(typename P::S* __restrict__ dst, const cx<double>* __restrict__
       dst[(mp * 6 + e) * g.vh + i] = Pack<P>::pack(cx<typename P::R>{(typename P::R)u.re,
                                                                       (typename P::R)u.im}, inv);
     }
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// Many right-hand sides (DWF): one launch applies the operator to every
+// active RHS.  A work-group holds ns sites x Ls slices x nr RHS (threadIdx.z);
+// the links staged in LDS serve all of them.  Vectors of RHS j start at
+// j * vs chunks (scales at j * vh * Ls); RHS j has its own mass and CG scalars
+// scal[j * MR_NSCAL + ...]; active[j] == 0 makes every kernel skip it, which
+// is how a converged RHS retires without changing any launch shape.
+// ---------------------------------------------------------------------------
+constexpr int MR_NSCAL = 8;
+enum { MR_RR = 0, MR_D = 1, MR_RRNEW = 2, MR_BETA = 3, MR_STOP = 4, MR_RRUPD = 5, MR_DELTA2 = 6,
+       MR_MAXIT = 7 };
+
+template <class P, class PX>
+struct MrArgs {
+  Fld<P> out;
+  CFld<P> in;
+  CFld<P> x0;
+  const typename P::S* U;
+  double* partial;  // [nrhs][gridDim.x]
+  Geom g;
+  int cb_in;
+  double two_a5, alpha;
+  const double* mass;  // [nrhs]
+  const int* active;   // [nrhs]
+  size_t vs;           // chunks per vector
+  int nrhs;
+  // CG epilogue (x in PX)
+  Fld<P> r;
+  CFld<P> p;
+  Fld<PX> x;
+  double* scal;  // [nrhs][MR_NSCAL]
+};
+
+template <class P>
+__device__ __forceinline__ Fld<P> fld_at(Fld<P> f, size_t vs, size_t vss, int j) {
+  return Fld<P>{f.d + j * vs, f.sc ? f.sc + j * vss : nullptr};
+}
+template <class P>
+__device__ __forceinline__ CFld<P> fld_at(CFld<P> f, size_t vs, size_t vss, int j) {
+  return CFld<P>{f.d + j * vs, f.sc ? f.sc + j * vss : nullptr};
+}
+
+// per-RHS (threadIdx.z) sum over a work-group, fixed order; valid in the
+// thread with threadIdx.x == threadIdx.y == 0 of each z
+__device__ __forceinline__ double zslice_sum(double v, double* red) {
+  const int nxy = blockDim.x * blockDim.y;
+  const int t = threadIdx.x + blockDim.x * threadIdx.y;
+  double* my = red + threadIdx.z * nxy;
+  my[t] = v;
+  __syncthreads();
+  int n = nxy;
+  while (n > 1) {
+    const int h = (n + 1) >> 1;
+    if (t < n - h) my[t] += my[t + h];
+    __syncthreads();
+    n = h;
+  }
+  return my[0];
+}
+
+template <class P, int DAG, int EPI, bool NORM, bool CG, class PX>
+__global__ void __launch_bounds__(512) k_dwf_mrhs(MrArgs<P, PX> a) {
+  typedef typename P::R R;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const Geom& g = a.g;
+  const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y, q = threadIdx.z;
+  const int nr = blockDim.z;
+  const int j = blockIdx.y * nr + q;
+  // the whole group leaves when none of its RHS is active (uniform branch)
+  bool any = false;
+  for (int k = 0; k < nr; ++k) {
+    const int jj = blockIdx.y * nr + k;
+    any = any || (jj < a.nrhs && a.active[jj]);
+  }
+  if (!any) return;
+  int* lnb = reinterpret_cast<int*>(smem_raw);
+  cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw + 2 * 8 * ns * 4);
+  double* red = reinterpret_cast<double*>(smem_raw + 2 * 8 * ns * 4 + 2 * 8 * ns * 9 * sizeof(cx<R>));
+  const int blk = xcd_block(g.nxcd);
+  const int i = blk * ns + il;
+  stage_links<P>(lu, lnb, a.U, g, blk, ns, 2, (1 + a.cb_in) & 1);
+  __syncthreads();
+  const bool act = j < a.nrhs && a.active[j] && i < g.vh;
+  const size_t vss = (size_t)g.vh * g.ls;
+  double nrm = 0.0;
+  if (act) {
+    const CFld<P> in = fld_at(a.in, a.vs, vss, j);
+    const size_t slice = (size_t)nch<P>() * g.vh;
+    cx<R> acc[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
+    hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
+              lu + ((s & 1) * 8 * ns + il) * 9, ns * 9, lnb + (s & 1) * 8 * ns + il, ns);
+    const int sm = s > 0 ? s - 1 : g.ls - 1, spl = s < g.ls - 1 ? s + 1 : 0;
+    const double nm = -2.0 * a.mass[j] * (0.5 * a.two_a5);
+    const R cm = (R)(s > 0 ? a.two_a5 : nm);
+    const R cp = (R)(s < g.ls - 1 ? a.two_a5 : nm);
+    const int lo_m = DAG ? 6 : 0, lo_p = DAG ? 0 : 6;
+    {
+      cx<R> vm[12], vp[12];
+      ld_spinor<P>(in.d + sm * slice, in.sc + (size_t)sm * g.vh, g.vh, i, vm);
+      ld_spinor<P>(in.d + spl * slice, in.sc + (size_t)spl * g.vh, g.vh, i, vp);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        acc[lo_m + c] = acc[lo_m + c] + cm * vm[lo_m + c];
+        acc[lo_p + c] = acc[lo_p + c] + cp * vp[lo_p + c];
+      }
+    }
+    if (EPI == 1) {
+      const CFld<P> x0 = fld_at(a.x0, a.vs, vss, j);
+      cx<R> xv[12];
+      ld_spinor<P>(x0.d + s * slice, x0.sc + (size_t)s * g.vh, g.vh, i, xv);
+      const R al = (R)a.alpha;
+#pragma unroll
+      for (int c = 0; c < 12; ++c) acc[c] = xv[c] + al * acc[c];
+      if (NORM) nrm = norm12(acc);
+      if (CG) {
+        const CgEpi<P, PX> cg{fld_at(a.r, a.vs, vss, j), fld_at(a.p, a.vs, vss, j),
+                              fld_at(a.x, a.vs * nch<PX>() / nch<P>(), vss, j),
+                              a.scal + (size_t)j * MR_NSCAL};
+        nrm = cg_epilogue<P, PX>(cg, g, s, i, acc);
+      }
+    }
+    if (!CG) {
+      const Fld<P> out = fld_at(a.out, a.vs, vss, j);
+      st_site<P>(out.d + s * slice, out.sc + (size_t)s * g.vh, g.vh, i, acc);
+    }
+  }
+  if (NORM || CG) {
+    const double t = zslice_sum(nrm, red);
+    if (il == 0 && s == 0 && j < a.nrhs) a.partial[(size_t)j * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// scal[j][slot] = sum of partial[j][0..n) for every RHS (one work-group each)
+__global__ void __launch_bounds__(1024)
+k_reduce_mrhs(const double* __restrict__ partial, int n, double* __restrict__ scal, int slot,
+              const int* __restrict__ active) {
+  __shared__ double sh[16];
+  const int j = blockIdx.x;
+  if (!active[j]) return;
+  const double* pj = partial + (size_t)j * n;
+  double a0 = 0.0, a1 = 0.0;
+  int k = threadIdx.x;
+  for (; k + 1024 < n; k += 2048) {
+    a0 += pj[k];
+    a1 += pj[k + 1024];
+  }
+  for (; k < n; k += 1024) a0 += pj[k];
+  const double t = block_sum(a0 + a1, sh);
+  if (threadIdx.x == 0) scal[(size_t)j * MR_NSCAL + slot] = t;
+}
+
+// end of a CG step for every active RHS: beta = rr_new / rr, rr = rr_new;
+// an RHS whose residual meets its stop condition, or has dropped by delta
+// since its last reliable update, pauses (active = 0) and raises flag = 1 for
+// the host; <p, A p> == 0 pauses it with flag = 2, reaching the iteration
+// cap with flag = 3.
+__global__ void __launch_bounds__(256)
+k_cg_scalars_mrhs(double* __restrict__ scal, int* __restrict__ active, int* __restrict__ flag,
+                  int* __restrict__ iters, int nrhs) {
+  for (int j = threadIdx.x; j < nrhs; j += blockDim.x) {
+    if (!active[j]) continue;
+    double* sc = scal + (size_t)j * MR_NSCAL;
+    iters[j] += 1;
+    if (sc[MR_D] == 0.0) {
+      active[j] = 0;
+      flag[j] = 2;
+      continue;
+    }
+    const double rrn = sc[MR_RRNEW];
+    sc[MR_BETA] = rrn / sc[MR_RR];
+    sc[MR_RR] = rrn;
+    if (rrn <= sc[MR_STOP] || rrn < sc[MR_DELTA2] * sc[MR_RRUPD]) {
+      active[j] = 0;
+      flag[j] = 1;
+    } else if (iters[j] >= (int)sc[MR_MAXIT]) {
+      active[j] = 0;
+      flag[j] = 3;
+    }
+  }
+}
+
+// p_j = r_j + beta_j p_j for every active RHS (blockIdx.y = RHS)
+template <class P>
+__global__ void __launch_bounds__(BLAS_THREADS)
+k_xpby_mrhs(Fld<P> p, CFld<P> r, const double* __restrict__ scal, const int* __restrict__ active,
+            size_t vs, Geom g) {
+  typedef typename P::R R;
+  const int j = blockIdx.y;
+  if (!active[j]) return;
+  const size_t vss = (size_t)g.vh * g.ls;
+  const Fld<P> pj = fld_at(p, vs, vss, j);
+  const CFld<P> rj = fld_at(r, vs, vss, j);
+  const R b = (R)scal[(size_t)j * MR_NSCAL + MR_BETA];
+  DWFK_SITE_LOOP(k, g) {
+    cx<R> pv[12], rv[12];
+    ld_site<P>(pv, CFld<P>{pj.d, pj.sc}, g, k);
+    ld_site<P>(rv, rj, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) pv[c] = rv[c] + b * pv[c];
+    st_site_k<P>(pj, g, k, pv);
   }
 }
 

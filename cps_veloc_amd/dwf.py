@@ -21,7 +21,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcps_dwf_gpu.so")
+# CPS_DWF_GPU_LIB selects another build of the same library (A/B timing)
+LIB_PATH = os.environ.get("CPS_DWF_GPU_LIB", os.path.join(_HERE, "libcps_dwf_gpu.so"))
 
 OK, BREAKDOWN, MAXITER, BAD_ARG, HIP_ERROR = 0, 1, 2, 3, 4
 DWF, MOBIUS = 0, 1
@@ -38,6 +39,8 @@ EXPORTS = (
     "dwfgpu_matinv", "dwfgpu_vec_create", "dwfgpu_vec_destroy", "dwfgpu_vec_upload",
     "dwfgpu_vec_download", "dwfgpu_matpcdagmatpc_dev", "dwfgpu_invcg_dev",
     "dwfgpu_invcg_mixed", "dwfgpu_invcg_mixed_dev",
+    "dwfgpu_mvec_create", "dwfgpu_mvec_destroy", "dwfgpu_mvec_upload", "dwfgpu_mvec_download",
+    "dwfgpu_matpcdagmatpc_mrhs_dev", "dwfgpu_invcg_mrhs_dev", "dwfgpu_mrhs_shape",
     "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
 )
 # every entry point of include/dwf_synth.h
@@ -104,6 +107,15 @@ def lib():
         L.dwfgpu_invcg_mixed_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, H, H,
                                              C.POINTER(C.c_double), C.POINTER(C.c_int),
                                              C.POINTER(C.c_int)]
+        L.dwfgpu_mvec_create.argtypes = [H, C.c_int, C.POINTER(H)]
+        L.dwfgpu_mvec_destroy.argtypes = [H]
+        L.dwfgpu_mvec_upload.argtypes = [H, C.c_int, _dp]
+        L.dwfgpu_mvec_download.argtypes = [H, C.c_int, _dp]
+        L.dwfgpu_matpcdagmatpc_mrhs_dev.argtypes = [H, C.c_int, _dp, H, H]
+        _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.dwfgpu_invcg_mrhs_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, C.c_int,
+                                            _dp, H, H, _dp, _ip, _ip, _ip]
+        L.dwfgpu_mrhs_shape.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.dwfgpu_time_matpcdagmatpc.argtypes = [H, C.c_int, C.c_double, H, H, C.c_int,
                                                 C.POINTER(C.c_double), _dp, _dp,
                                                 C.POINTER(C.c_int)]
@@ -149,6 +161,38 @@ class DeviceVector:
     def free(self):
         if self.h:
             lib().dwfgpu_vec_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class MultiVector:
+    """nrhs CG half vectors resident in HBM (dwfgpu_mvec_t, FP64 handle)."""
+
+    def __init__(self, op: "_DiracOpGpu", nrhs, hosts=None):
+        self.op = op
+        self.nrhs = int(nrhs)
+        self.h = C.c_void_p()
+        _chk(lib().dwfgpu_mvec_create(op.h, self.nrhs, C.byref(self.h)), "dwfgpu_mvec_create")
+        if hosts is not None:
+            for j, x in enumerate(hosts):
+                self.upload(j, x)
+
+    def upload(self, j, host):
+        _chk(lib().dwfgpu_mvec_upload(self.h, j, _arr(host)), "dwfgpu_mvec_upload")
+
+    def download(self, j):
+        out = np.empty(self.op.n)
+        _chk(lib().dwfgpu_mvec_download(self.h, j, out), "dwfgpu_mvec_download")
+        return out
+
+    def free(self):
+        if self.h:
+            lib().dwfgpu_mvec_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):
@@ -252,6 +296,36 @@ class _DiracOpGpu:
                                                src.h, C.byref(tr), C.byref(it), C.byref(nr)),
                   "InvCgMixed_dev", allow=(OK, MAXITER))
         return it.value, tr.value, st, nr.value
+
+    # --- many right-hand sides (DWF) ----------------------------------------
+    def multi_vector(self, nrhs, hosts=None):
+        return MultiVector(self, nrhs, hosts)
+
+    def MatPcDagMatPc_mrhs_dev(self, out: MultiVector, inp: MultiVector, masses,
+                               precision="f64"):
+        m = _arr(masses)
+        _chk(lib().dwfgpu_matpcdagmatpc_mrhs_dev(self.h, _PREC[precision], m, out.h, inp.h),
+             "MatPcDagMatPc_mrhs_dev")
+
+    def InvCgMrhs_dev(self, sol: MultiVector, src: MultiVector, cg_arg: CgArg, masses,
+                      sloppy="f32", delta=0.1, eager=False):
+        """Batched reliable-update CG; returns per-RHS arrays
+        (iters, true_res, status, reliable_updates)."""
+        n = sol.nrhs
+        tr = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        nr = np.zeros(n, dtype=np.int32)
+        stv = np.zeros(n, dtype=np.int32)
+        a = cg_arg.c()
+        _chk(lib().dwfgpu_invcg_mrhs_dev(self.h, C.byref(a), _PREC[sloppy], delta,
+                                         1 if eager else 0, _arr(masses), sol.h, src.h, tr, it,
+                                         nr, stv), "InvCgMrhs_dev")
+        return it, tr, stv, nr
+
+    def mrhs_shape(self):
+        ns, nr = C.c_int(), C.c_int()
+        _chk(lib().dwfgpu_mrhs_shape(self.h, C.byref(ns), C.byref(nr)), "mrhs_shape")
+        return ns.value, nr.value
 
     def time_MatPcDagMatPc(self, out: DeviceVector, inp: DeviceVector, mass, reps,
                            precision=None):

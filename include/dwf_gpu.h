@@ -132,6 +132,36 @@ int dwfgpu_invcg_mixed_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, dou
                            dwfgpu_vec_t sol, dwfgpu_vec_t src, double *true_res, int *iters,
                            int *reliable_updates);
 
+/* ---- many right-hand sides (DWF) ---------------------------------------- */
+/* nrhs FP64 half vectors on the device (an FP64 handle's layout), filled and
+ * read one RHS at a time in CPS half-vector order.                           */
+typedef struct dwfgpu_mvec_s *dwfgpu_mvec_t;
+int dwfgpu_mvec_create(dwfgpu_t h, int nrhs, dwfgpu_mvec_t *out);
+int dwfgpu_mvec_destroy(dwfgpu_mvec_t v);
+int dwfgpu_mvec_upload(dwfgpu_mvec_t v, int j, const double *host_cps_order);
+int dwfgpu_mvec_download(dwfgpu_mvec_t v, int j, double *host_cps_order);
+/* MatPcDagMatPc of every RHS j with mass masses[j], applied in `precision`
+ * by the batched operator (one launch per hopping pass for all RHS, links
+ * staged once per work-group for all of them); DWF handles only.  The
+ * per-RHS loop over DiracOpDwf::MatPcDagMatPc (d_op_dwf.C:174-200) that CPS
+ * callers such as QPropW (QPropW.C:1236) run, in one pass.                   */
+int dwfgpu_matpcdagmatpc_mrhs_dev(dwfgpu_t h, int precision, const double *masses,
+                                  dwfgpu_mvec_t out, dwfgpu_mvec_t in);
+/* InvCg's solve (inv_cg.C:84-448 stop rule per RHS) for every RHS j with its
+ * own mass, in one batched CG: `sloppy` (F32, BF16 or I16) steps with FP64
+ * reliable updates as in dwfgpu_invcg_mixed; an RHS that converges retires on
+ * the device while the others go on; the CG step is captured in a HIP graph
+ * and replayed (flags & DWFGPU_MRHS_EAGER: plain launches instead).  Per RHS:
+ * true_res[j], iters[j] (sloppy steps + 1), reliable_updates[j], status[j]
+ * (DWFGPU_OK, DWFGPU_MAXITER or DWFGPU_BREAKDOWN).  The launch shape of the
+ * batched operator is chosen on first use by timing the candidates.         */
+#define DWFGPU_MRHS_EAGER 1
+int dwfgpu_invcg_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, double delta,
+                          int flags, const double *masses, dwfgpu_mvec_t sol, dwfgpu_mvec_t src,
+                          double *true_res, int *iters, int *reliable_updates, int *status);
+/* the launch shape (sites, RHS per work-group) the last batched call used */
+int dwfgpu_mrhs_shape(dwfgpu_t h, int *ns, int *nr);
+
 /* ---- instrumentation (the role of CPS's print_flops / CGflops) --------- */
 /* Times `reps` MatPcDagMatPc applications on the handle's stream with HIP
  * events around each of its kernel launches.  precision: the handle's API
