@@ -39,6 +39,10 @@ WORKLOADS = {
                                     eps=0.3, b=1.5, c=0.5),
     "dwf_16x16x16x32_ls16": dict(action="dwf", dims=(16, 16, 16, 32), ls=16, mass=0.01, eps=0.3),
     "dwf_8x8x8x8_ls8": dict(action="dwf", dims=(8, 8, 8, 8), ls=8, mass=0.04, eps=0.3),
+    # 24 RHS: 12 spin-colour point sources at the origin (P+ on s = 0, P- on
+    # s = Ls-1) and 12 complex Z2 noise sources, masses 0.005 / 0.01 / 0.04
+    "dwf_16x16x16x32_ls16_mrhs24": dict(action="dwf", dims=(16, 16, 16, 32), ls=16, mass=0.01,
+                                        eps=0.3, nrhs=24, masses=(0.005, 0.01, 0.04)),
 }
 
 
@@ -69,6 +73,94 @@ def cpu_baseline(wl, U, src, seconds_hint=True):
     return dt, cores
 
 
+def mrhs_sources(op, wl, kappa, seed=99):
+    """CG sources of the batched workload: full 5D sources (point or Z2),
+    prepared as DiracOpDwf::MatInv does (d_op_dwf.C:357-415):
+    b = MatPcDag(eta_odd + kappa Dslash(eta_even))."""
+    rng = np.random.default_rng(seed)
+    n, ls = op.n, wl["ls"]
+    vh = int(np.prod(wl["dims"])) // 2
+    srcs, masses = [], []
+    for j in range(wl["nrhs"]):
+        mass = wl["masses"][j % len(wl["masses"])]
+        ev = np.zeros((ls, vh, 4, 3, 2))
+        od = np.zeros((ls, vh, 4, 3, 2))
+        if j < 12:                   # spin-colour point source at the origin
+            spin, col = divmod(j, 3)
+            if spin < 2:
+                ev[0, 0, spin, col, 0] = 1.0          # P+ -> s = 0 (5D-even half)
+            else:
+                od[ls - 1, 0, spin, col, 0] = 1.0     # P- -> s = Ls-1 (5D-odd half)
+        else:                        # complex Z2 noise on every site
+            ev = rng.choice([-1.0, 1.0], size=ev.shape) / np.sqrt(2.0)
+            od = rng.choice([-1.0, 1.0], size=od.shape) / np.sqrt(2.0)
+        t = np.zeros(n)
+        op.Dslash(t, ev.reshape(-1), mass, 0, 0)
+        t = od.reshape(-1) + kappa * t
+        b = np.zeros(n)
+        op.MatPcDag(b, t, mass)
+        srcs.append(b)
+        masses.append(mass)
+    return srcs, np.array(masses)
+
+
+def bench_mrhs(args, wl):
+    """Batched solve of all RHS vs the same RHS one at a time."""
+    from cps_veloc_amd import lattice as lat
+    from cps_veloc_amd.dwf import CgArg, DiracOpDwfGpu
+    dims, ls = wl["dims"], wl["ls"]
+    log(f"[bench] workload {args.workload}: gauge and {wl['nrhs']} sources")
+    U = lat.random_gauge(dims, wl["eps"], 1234)
+    op = DiracOpDwfGpu(U, dims, ls, m5=1.8)
+    kappa = 1.0 / (2.0 * (4.0 + 1.0 - 1.8))
+    srcs, masses = mrhs_sources(op, wl, kappa)
+    nrhs = len(srcs)
+    vsrc, vsol = op.multi_vector(nrhs, srcs), op.multi_vector(nrhs)
+    arg = CgArg(0.0, args.max_iter, 1e-10)
+    # warm-up: the autotuner and graph set-up of the batched path
+    op.InvCgMrhs_dev(vsol, vsrc, CgArg(0.0, 3, 1e-10), masses, args.mrhs_sloppy, args.delta)
+    for j in range(nrhs):
+        vsol.upload(j, np.zeros(op.n))
+    log("[bench] batched solve")
+    t0 = time.perf_counter()
+    it, tr, st, nrel = op.InvCgMrhs_dev(vsol, vsrc, arg, masses, args.mrhs_sloppy, args.delta)
+    t_batch = time.perf_counter() - t0
+    ns, nr = op.mrhs_shape()
+    log(f"[bench] batched: {t_batch:.2f}s, iters {it.tolist()}, shape ns={ns} nr={nr}")
+    xb = [vsol.download(j) for j in range(nrhs)]
+    # the same RHS one at a time with the single-RHS mixed solver
+    v1, s1 = op.vector(), op.vector()
+    t_seq, it_seq, err = 0.0, [], []
+    for j in range(nrhs):
+        s1.upload(srcs[j])
+        v1.upload(np.zeros(op.n))
+        a = CgArg(masses[j], args.max_iter, 1e-10)
+        t0 = time.perf_counter()
+        i1, tr1, st1, _ = op.InvCgMixed_dev(v1, s1, a, args.mrhs_sloppy, args.delta)
+        t_seq += time.perf_counter() - t0
+        it_seq.append(i1)
+        x1 = v1.download()
+        err.append(float(np.linalg.norm(xb[j] - x1) / np.linalg.norm(x1)))
+    log(f"[bench] one at a time: {t_seq:.2f}s, iters {it_seq}")
+    out = {
+        "metric": "batched 24-RHS DWF solve to 1e-10: right-hand sides solved per second",
+        "value": round(nrhs / t_batch, 4), "unit": "RHS/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": round(t_batch * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": args.mrhs_sloppy + "+f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "lattice": list(dims), "ls": ls,
+                   "masses": masses.tolist(), "stop_rsd": 1e-10, "delta": args.delta,
+                   "sloppy": args.mrhs_sloppy, "shape": {"ns": ns, "nr": nr}},
+        "batched": {"seconds": round(t_batch, 3), "iters": it.tolist(),
+                    "true_res": tr.tolist(), "status": st.tolist(),
+                    "reliable_updates": nrel.tolist()},
+        "one_at_a_time": {"seconds": round(t_seq, 3), "iters": it_seq},
+        "speedup_batched_vs_one_at_a_time": round(t_seq / t_batch, 3),
+        "max_rel_diff_batched_vs_single": max(err),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +175,7 @@ def main():
                     help="sloppy precisions of the reliable-update CG solves")
     ap.add_argument("--delta", type=float, default=0.1,
                     help="reliable update when |r| drops by this factor")
+    ap.add_argument("--mrhs-sloppy", default="f32", choices=["f32", "bf16", "i16"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU "
@@ -93,6 +186,9 @@ def main():
     from cps_veloc_amd.dwf import CgArg, DiracOpDwfGpu, DiracOpMobiusGpu
 
     wl = WORKLOADS[args.workload]
+    if "nrhs" in wl:
+        bench_mrhs(args, wl)
+        return
     dims, ls, mass = wl["dims"], wl["ls"], wl["mass"]
     vh = int(np.prod(dims)) // 2
     log(f"[bench] workload {args.workload}: generating inputs")
