@@ -295,7 +295,8 @@ def main():
         "sloppy_matpcdagmatpc": sloppy,
         "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None if prof is None else prof.get("traffic_bytes"),
+                     "traffic": None if prof is None else
+                     prof.get("kernels", {}).get(KERNELS[wl["action"]][hop], {}).get("traffic_bytes"),
                      "kernel": KERNELS[wl["action"]][hop],
                      "algorithmic_bytes": bytes_k[hop]},
         "cpu_baseline": cpu,

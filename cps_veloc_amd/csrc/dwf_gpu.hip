@@ -3,8 +3,10 @@
 // the reliable-update mixed-precision CG.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
+#include <random>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -13,7 +15,9 @@
 #include <vector>
 
 #include "dwf_gpu.h"
+#include "dwf_synth.h"
 #include "dwf_kernels.hpp"
+#include "ptc_kernels.hpp"
 #include "roofline.h"
 
 using dwfk::cx;
@@ -93,6 +97,7 @@ struct dwfgpu_s {
   // sloppy precision and RHS count (0: not tuned yet)
   int mr_ns[NPREC] = {0, 0, 0, 0}, mr_nr[NPREC] = {0, 0, 0, 0}, mr_nrhs[NPREC] = {0, 0, 0, 0};
   int mr_last_ns = 0, mr_last_nr = 0;
+  struct PtcNet* net = nullptr;  // learned preconditioner (dwfgpu_ptc_*)
 };
 
 struct dwfgpu_vec_s {
@@ -810,7 +815,10 @@ int time_mdagm(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int re
   return st;
 }
 
+void ptc_free(PtcNet* n);
 void free_all(dwfgpu_s* h) {
+  if (h->net) ptc_free(h->net);
+  h->net = nullptr;
   for (int p = 0; p < NPREC; ++p) {
     PrecFields& F = h->f[p];
     if (F.U) (void)hipFree(F.U);
@@ -854,6 +862,494 @@ const DVec& api_work(dwfgpu_s* h, int k) { return h->f[h->prec].work[k]; }
 
 }  // namespace
 
+
+
+// ===========================================================================
+// Learned preconditioner: the parallel-transport network (ptc_kernels.hpp),
+// its training on the GPU and the flexible CG that applies it
+// ===========================================================================
+struct PtcNet {
+  int ls = 0, hid = 16;
+  std::vector<std::complex<float>> W[3];  // [9][cout][cin][4][4]
+  std::vector<double> m[3], v[3];         // Adam moments of (re, im)
+  long adam_t = 0;
+  uint16_t* A[3] = {nullptr, nullptr, nullptr};     // bf16 real forms, [9][M'][K']
+  uint16_t* Aadj[3] = {nullptr, nullptr, nullptr};  // of the adjoint layers
+  uint32_t* act[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint32_t* gact[2] = {nullptr, nullptr};
+  float* gwpart = nullptr;
+  int nwg = 64;
+  cx<float>* G[3] = {nullptr, nullptr, nullptr};
+  int cin(int k) const { return k == 0 ? ls : hid; }
+  int cout(int k) const { return k == 2 ? ls : hid; }
+  size_t nw(int k) const { return (size_t)ptck::NDIR * cout(k) * cin(k) * 16; }
+};
+
+namespace {
+
+void ptc_free(PtcNet* n) {
+  if (!n) return;
+  for (int k = 0; k < 3; ++k) {
+    if (n->A[k]) (void)hipFree(n->A[k]);
+    if (n->Aadj[k]) (void)hipFree(n->Aadj[k]);
+    if (n->G[k]) (void)hipFree(n->G[k]);
+  }
+  for (auto p : n->act)
+    if (p) (void)hipFree(p);
+  for (auto p : n->gact)
+    if (p) (void)hipFree(p);
+  if (n->gwpart) (void)hipFree(n->gwpart);
+  delete n;
+}
+
+uint16_t bf16_host(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+// A_d = [[Re W_d, -Im W_d], [Im W_d, Re W_d]] (rows (part, o, a), columns
+// (part, i, b)) for every d, from W[d][o][i][a][b]; adj: the real form of
+// the adjoint layer, W'_{d'}[i][o][b][a] = conj(W_d[o][i][a][b]), d' = -d
+void ptc_real_form(const std::vector<std::complex<float>>& W, int cin, int cout, bool adj,
+                   std::vector<uint16_t>& out) {
+  const int ci = adj ? cout : cin, co = adj ? cin : cout;
+  const int MP = 8 * co, KP = 8 * ci;
+  out.assign((size_t)ptck::NDIR * MP * KP, 0);
+  for (int d = 0; d < ptck::NDIR; ++d) {
+    const int ds = adj ? (d == 0 ? 0 : (d <= 4 ? d + 4 : d - 4)) : d;  // source direction
+    for (int o = 0; o < co; ++o)
+      for (int i = 0; i < ci; ++i)
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b) {
+            std::complex<float> w;
+            if (!adj)
+              w = W[(((size_t)(ds * cout + o) * cin + i) * 4 + a) * 4 + b];
+            else  // W'[o'=i][i'=o][a'=b][b'=a] = conj(W[o][i][a][b])
+              w = std::conj(W[(((size_t)(ds * cout + i) * cin + o) * 4 + b) * 4 + a]);
+            const int mre = o * 4 + a, mim = co * 4 + o * 4 + a;
+            const int kre = i * 4 + b, kim = ci * 4 + i * 4 + b;
+            uint16_t* Ad = out.data() + (size_t)d * MP * KP;
+            Ad[mre * KP + kre] = bf16_host(w.real());
+            Ad[mre * KP + kim] = bf16_host(-w.imag());
+            Ad[mim * KP + kre] = bf16_host(w.imag());
+            Ad[mim * KP + kim] = bf16_host(w.real());
+          }
+  }
+}
+
+int ptc_upload(dwfgpu_s* h) {
+  PtcNet* n = h->net;
+  std::vector<uint16_t> buf;
+  for (int k = 0; k < 3; ++k) {
+    ptc_real_form(n->W[k], n->cin(k), n->cout(k), false, buf);
+    HIPCHK(hipMemcpyAsync(n->A[k], buf.data(), buf.size() * 2, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    ptc_real_form(n->W[k], n->cin(k), n->cout(k), true, buf);
+    HIPCHK(hipMemcpyAsync(n->Aadj[k], buf.data(), buf.size() * 2, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+  }
+  return DWFGPU_OK;
+}
+
+template <int CIN, int COUT>
+int ptc_fwd_t(dwfgpu_s* h, uint32_t* out, const uint32_t* in, const uint16_t* A) {
+  constexpr size_t lds = ptck::fwd_lds<CIN, COUT>();
+  if (lds > (size_t)h->max_lds) {
+    set_err("ptc: layer %dx%d needs %zu bytes of LDS", CIN, COUT, lds);
+    return DWFGPU_BAD_ARG;
+  }
+  hipLaunchKernelGGL((ptck::k_ptc_fwd<CIN, COUT>), dim3(2 * h->g.vh / ptck::NT), dim3(ptck::NTHR),
+                     lds, h->st, out, in, A, (const typename P32::S*)h->f[DWFGPU_F32].U, h->g);
+  HIPCHK(hipGetLastError());
+  return DWFGPU_OK;
+}
+
+template <int CIN, int COUT>
+int ptc_gw_t(dwfgpu_s* h, float* part, int nwg, const uint32_t* gout, const uint32_t* in) {
+  constexpr size_t lds = ptck::gw_lds<CIN, COUT>();
+  if (lds > (size_t)h->max_lds) {
+    set_err("ptc: gradient %dx%d needs %zu bytes of LDS", CIN, COUT, lds);
+    return DWFGPU_BAD_ARG;
+  }
+  hipLaunchKernelGGL((ptck::k_ptc_gw<CIN, COUT>), dim3(nwg, ptck::NDIR), dim3(ptck::NTHR), lds,
+                     h->st, part, gout, in, (const typename P32::S*)h->f[DWFGPU_F32].U, h->g);
+  HIPCHK(hipGetLastError());
+  return DWFGPU_OK;
+}
+
+// the channel pairs the kernels are built for: Ls in {4, 8, 12, 16}, H = 16
+#define PTC_PAIRS(X) X(4, 16) X(16, 4) X(8, 16) X(16, 8) X(12, 16) X(16, 12) X(16, 16)
+
+int ptc_fwd(dwfgpu_s* h, int cin, int cout, uint32_t* out, const uint32_t* in, const uint16_t* A) {
+#define PTC_F(CI, CO) \
+  if (cin == CI && cout == CO) return ptc_fwd_t<CI, CO>(h, out, in, A);
+  PTC_PAIRS(PTC_F)
+#undef PTC_F
+  set_err("ptc: no layer kernel for %d -> %d channels", cin, cout);
+  return DWFGPU_BAD_ARG;
+}
+
+int ptc_gw(dwfgpu_s* h, int cin, int cout, float* part, int nwg, const uint32_t* gout,
+           const uint32_t* in) {
+#define PTC_G(CI, CO) \
+  if (cin == CI && cout == CO) return ptc_gw_t<CI, CO>(h, part, nwg, gout, in);
+  PTC_PAIRS(PTC_G)
+#undef PTC_G
+  set_err("ptc: no gradient kernel for %d -> %d channels", cin, cout);
+  return DWFGPU_BAD_ARG;
+}
+
+// z = |r| net(r/|r|) on sloppy vectors (z must differ from r)
+template <class P>
+int ptc_apply(dwfgpu_s* h, const DVec& z, const DVec& r, double* rnorm_out = nullptr) {
+  PtcNet* n = h->net;
+  double rr;
+  TRY(Ops<P>::norm(h, r, &rr));
+  const double rn = std::sqrt(rr);
+  if (rnorm_out) *rnorm_out = rn;
+  if (rn == 0.0) return Ops<P>::copy(h, z, r);  // r = 0: z = 0
+  const int action = h->action == DWFGPU_DWF ? 0 : 1;
+  hipLaunchKernelGGL((ptck::k_ptc_embed<P>), dim3(2048), dim3(256), 0, h->st, n->act[0],
+                     cfld<P>(r), h->g, action, (float)(1.0 / rn));
+  HIPCHK(hipGetLastError());
+  for (int k = 0; k < 3; ++k) TRY(ptc_fwd(h, n->cin(k), n->cout(k), n->act[k + 1], n->act[k], n->A[k]));
+  hipLaunchKernelGGL((ptck::k_ptc_extract<P>), dim3(2048), dim3(256), 0, h->st, fld<P>(z),
+                     n->act[3], h->g, action, (float)rn);
+  HIPCHK(hipGetLastError());
+  return DWFGPU_OK;
+}
+
+// loss of one (r, e) pair (FP32 vectors) and G[k] (+)= dL/dW[k]
+int ptc_grad_pair(dwfgpu_s* h, const DVec& r, const DVec& e, bool accumulate, double* loss) {
+  PtcNet* n = h->net;
+  const int action = h->action == DWFGPU_DWF ? 0 : 1;
+  double rr, ee;
+  TRY(Ops<P32>::norm(h, r, &rr));
+  TRY(Ops<P32>::norm(h, e, &ee));
+  const double rn = std::sqrt(rr);
+  if (rn == 0.0 || ee == 0.0) {
+    set_err("ptc: empty training pair");
+    return DWFGPU_BAD_ARG;
+  }
+  const double nt = ee / rr;  // |e/|r||^2
+  hipLaunchKernelGGL((ptck::k_ptc_embed<P32>), dim3(2048), dim3(256), 0, h->st, n->act[0],
+                     cfld<P32>(r), h->g, action, (float)(1.0 / rn));
+  HIPCHK(hipGetLastError());
+  for (int k = 0; k < 3; ++k) TRY(ptc_fwd(h, n->cin(k), n->cout(k), n->act[k + 1], n->act[k], n->A[k]));
+  hipLaunchKernelGGL(ptck::k_ptc_loss, dim3(dwfk::BLAS_BLOCKS), dim3(256), 0, h->st, n->gact[0],
+                     n->act[3], cfld<P32>(e), (float)(1.0 / rn), h->g, action, (float)(1.0 / nt),
+                     h->partial);
+  HIPCHK(hipGetLastError());
+  double dd;
+  TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &dd));
+  if (loss) *loss = dd / nt;
+  // backward: layer 2, 1, 0 (gact ping-pong)
+  int cur = 0;
+  for (int k = 2; k >= 0; --k) {
+    TRY(ptc_gw(h, n->cin(k), n->cout(k), n->gwpart, n->nwg, n->gact[cur], n->act[k]));
+    hipLaunchKernelGGL(ptck::k_ptc_gw_reduce, dim3(128), dim3(256), 0, h->st, n->G[k], n->gwpart,
+                       n->nwg, n->cin(k), n->cout(k), accumulate ? 1 : 0);
+    HIPCHK(hipGetLastError());
+    if (k > 0) {
+      TRY(ptc_fwd(h, n->cout(k), n->cin(k), n->gact[1 - cur], n->gact[cur], n->Aadj[k]));
+      cur = 1 - cur;
+    }
+  }
+  return DWFGPU_OK;
+}
+
+// one Adam step with the gradients accumulated over `count` pairs
+int ptc_adam(dwfgpu_s* h, int count, double lr) {
+  PtcNet* n = h->net;
+  const double b1 = 0.9, b2 = 0.999, eps = 1e-8;
+  ++n->adam_t;
+  const double c1 = 1.0 - std::pow(b1, (double)n->adam_t), c2 = 1.0 - std::pow(b2, (double)n->adam_t);
+  for (int k = 0; k < 3; ++k) {
+    std::vector<std::complex<float>> G(n->nw(k));
+    HIPCHK(hipMemcpyAsync(G.data(), n->G[k], G.size() * sizeof(cx<float>), hipMemcpyDeviceToHost,
+                          h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (size_t q = 0; q < G.size(); ++q) {
+      const double g2[2] = {G[q].real() / count, G[q].imag() / count};
+      double w2[2] = {n->W[k][q].real(), n->W[k][q].imag()};
+      for (int c = 0; c < 2; ++c) {
+        double& m = n->m[k][2 * q + c];
+        double& v = n->v[k][2 * q + c];
+        m = b1 * m + (1.0 - b1) * g2[c];
+        v = b2 * v + (1.0 - b2) * g2[c] * g2[c];
+        w2[c] -= lr * (m / c1) / (std::sqrt(v / c2) + eps);
+      }
+      n->W[k][q] = std::complex<float>((float)w2[0], (float)w2[1]);
+    }
+  }
+  return ptc_upload(h);
+}
+
+int ptc_ensure(dwfgpu_s* h) {
+  if (h->net) return DWFGPU_OK;
+  const int ls = h->ls;
+  if (!(ls == 4 || ls == 8 || ls == 12 || ls == 16)) {
+    set_err("ptc: the network is built for Ls in {4, 8, 12, 16} (got %d)", ls);
+    return DWFGPU_BAD_ARG;
+  }
+  if (h->g.vh % ptck::NT) {
+    set_err("ptc: V4/2 must be a multiple of %d", ptck::NT);
+    return DWFGPU_BAD_ARG;
+  }
+  TRY(ensure_fields<P32>(h));  // compressed FP32 links
+  PtcNet* n = new PtcNet;
+  n->ls = ls;
+  h->net = n;
+  int st = DWFGPU_OK;
+  const size_t V = 2 * (size_t)h->g.vh;
+  const int cmax = std::max(ls, n->hid);
+  for (int k = 0; k < 3 && !st; ++k) {
+    n->W[k].assign(n->nw(k), std::complex<float>(0.f, 0.f));
+    n->m[k].assign(2 * n->nw(k), 0.0);
+    n->v[k].assign(2 * n->nw(k), 0.0);
+    const size_t abytes = (size_t)ptck::NDIR * 64 * n->cin(k) * n->cout(k) * 2;
+    st = alloc(h, (void**)&n->A[k], abytes);
+    if (!st) st = alloc(h, (void**)&n->Aadj[k], abytes);
+    if (!st) st = alloc(h, (void**)&n->G[k], n->nw(k) * sizeof(cx<float>));
+  }
+  const int chans[4] = {ls, n->hid, n->hid, ls};
+  for (int k = 0; k < 4 && !st; ++k) st = alloc(h, (void**)&n->act[k], V * chans[k] * 12 * 4);
+  for (int k = 0; k < 2 && !st; ++k) st = alloc(h, (void**)&n->gact[k], V * cmax * 12 * 4);
+  if (!st)
+    st = alloc(h, (void**)&n->gwpart,
+               (size_t)ptck::NDIR * n->nwg * 64 * n->hid * n->hid * sizeof(float));
+  if (st) {
+    ptc_free(n);
+    h->net = nullptr;
+  }
+  return st;
+}
+
+// default weights: a damped local copy plus small random hops, so that the
+// untrained network is a scaled identity (z = 0.5 r) and training starts
+// from a valid preconditioner
+void ptc_init_weights(PtcNet* n, unsigned long long seed, double scale) {
+  std::mt19937_64 rng(seed);
+  std::normal_distribution<double> nd(0.0, 1.0);
+  for (int k = 0; k < 3; ++k) {
+    const int ci = n->cin(k), co = n->cout(k);
+    for (size_t q = 0; q < n->nw(k); ++q)
+      n->W[k][q] = std::complex<float>((float)(scale * nd(rng)), (float)(scale * nd(rng)));
+    // d = 0, o = i (channels that exist on both sides), a = b: a real diagonal
+    const double diag = k == 2 ? 0.5 : 1.0;
+    for (int o = 0; o < std::min(ci, co); ++o)
+      for (int a = 0; a < 4; ++a)
+        n->W[k][(((size_t)(0 * co + o) * ci + o) * 4 + a) * 4 + a] += std::complex<float>((float)diag, 0.f);
+  }
+  for (int k = 0; k < 3; ++k) {
+    std::fill(n->m[k].begin(), n->m[k].end(), 0.0);
+    std::fill(n->v[k].begin(), n->v[k].end(), 0.0);
+  }
+  n->adam_t = 0;
+}
+
+}  // namespace
+
+
+namespace {
+
+template <class P>
+int dot(dwfgpu_s* h, const DVec& x, const DVec& y, double* out) {
+  hipLaunchKernelGGL((ptck::k_dot<P>), blas_grid(), blas_block(), 0, h->st, cfld<P>(x), cfld<P>(y),
+                     h->g, h->partial);
+  HIPCHK(hipGetLastError());
+  return fetch_sum(h, dwfk::BLAS_BLOCKS, out);
+}
+
+// Training pairs from one solve of A x = b: FP32 CG from x = 0 keeps x_k at
+// iterations stride, 2 stride, ...; with x* from a reliable-update solve,
+// e_k = x* - x_k and r_k = A e_k (FP32 operator), so A e_k = r_k holds for
+// the stored vectors exactly up to FP32 rounding.  Late CG residuals are
+// rich in the low modes a preconditioner has to capture.
+int ptc_pairs(dwfgpu_s* h, double mass, const DVec& b64, int stride, int want,
+              std::vector<DVec>& R, std::vector<DVec>& E, int* made) {
+  // pairs R/E[0..want) from the CG snapshots (filled below)
+  typedef Ops<P32> L;
+  *made = 0;
+  DVec x64, x32, r32, p32;
+  int st = alloc_vec<P64>(h, &x64);
+  if (!st) st = alloc_vec<P32>(h, &x32);
+  if (!st) st = alloc_vec<P32>(h, &r32);
+  if (!st) st = alloc_vec<P32>(h, &p32);
+  auto done = [&](int code) {
+    free_vec(&x64);
+    free_vec(&x32);
+    free_vec(&r32);
+    free_vec(&p32);
+    return code;
+  };
+  if (st) return done(st);
+  const dwfgpu_cg_arg a{mass, 20000, 1e-9, 1e-9};
+  double tr;
+  int it, nrel;
+  st = invcg_mixed<P32>(h, &a, 0.1, x64, b64, &tr, &it, &nrel);
+  if (st != DWFGPU_OK && st != DWFGPU_MAXITER) return done(st);
+  // FP32 CG with snapshots
+  hipLaunchKernelGGL((dwfk::k_convert<P32, P64>), blas_grid(), blas_block(), 0, h->st,
+                     fld<P32>(r32), cfld<P64>(b64), h->g);
+  if (hipGetLastError() != hipSuccess) return done(DWFGPU_HIP_ERROR);
+  if ((st = L::copy(h, p32, r32))) return done(st);
+  if ((st = L::scale(h, x32, 0.0))) return done(st);
+  double rr;
+  if ((st = L::norm(h, r32, &rr))) return done(st);
+  // dscal[0] = rr for the fused step
+  HIPCHK(hipMemcpyAsync(h->dscal, &rr, sizeof(double), hipMemcpyHostToDevice, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  const double stp = rr * 1e-18;
+  int k = 0;
+  for (int itr = 1; itr <= stride * want && k < want; ++itr) {
+    double rrn, d;
+    if ((st = L::template mdagm_cg<P32>(h, mass, p32, r32, x32, &rrn, &d))) return done(st);
+    if (d == 0.0 || rrn <= stp) break;
+    if (itr % stride == 0) {
+      if ((st = L::copy(h, E[k], x32))) return done(st);
+      ++k;
+    }
+    if ((st = L::axpby(h, p32, r32, 1.0, rrn / rr))) return done(st);
+    rr = rrn;
+  }
+  // e_k = x* - x_k, r_k = A e_k
+  hipLaunchKernelGGL((dwfk::k_convert<P32, P64>), blas_grid(), blas_block(), 0, h->st,
+                     fld<P32>(x32), cfld<P64>(x64), h->g);
+  if (hipGetLastError() != hipSuccess) return done(DWFGPU_HIP_ERROR);
+  for (int j = 0; j < k; ++j) {
+    if ((st = L::axpby(h, E[j], x32, 1.0, -1.0))) return done(st);
+    if ((st = L::mdagm(h, mass, R[j], E[j], nullptr))) return done(st);
+  }
+  *made = k;
+  return done(DWFGPU_OK);
+}
+
+// Flexible CG (truncated to one previous direction) on A = MatPcDagMatPc
+// with z = net(r) as preconditioner, sloppy vectors in PS and FP64
+// reliable updates; the search direction restarts at every reliable update.
+// The net is switched off (plain CG from then on) when (r, net(r)) <= 0 or
+// is not finite, or when the residual stalls for `stall` steps.
+template <class PS>
+int invfcg_ptc(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec& sol,
+               const DVec& src, double* true_res, int* iters, int* nrel_out, int* net_steps) {
+  typedef Ops<P64> H;
+  typedef Ops<PS> L;
+  TRY(ensure_work<P64>(h, 2));
+  TRY(ensure_work<PS>(h, 4));
+  TRY(ensure_work<P32>(h, 5));
+  const DVec& r = h->f[DWFGPU_F64].work[0];
+  const DVec& ax = h->f[DWFGPU_F64].work[1];
+  const DVec& rs = h->f[pidx<PS>()].work[0];
+  const DVec& ps = h->f[pidx<PS>()].work[1];
+  const DVec& qs = h->f[pidx<PS>()].work[2];
+  const DVec& z = h->f[pidx<PS>()].work[3];
+  const DVec& xs = h->f[DWFGPU_F32].work[4];
+  const double mass = arg->mass;
+  double sn;
+  TRY(H::norm(h, src, &sn));
+  const double stp = sn * arg->stop_rsd * arg->stop_rsd;
+  TRY(H::mdagm(h, mass, ax, sol, nullptr));
+  TRY(H::axpy_to(h, r, ax, src, -1.0, true));
+  double rr;
+  TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
+  hipLaunchKernelGGL((dwfk::k_convert<PS, P64>), blas_grid(), blas_block(), 0, h->st, fld<PS>(rs),
+                     cfld<P64>(r), h->g);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemsetAsync(xs.d, 0, h->n / 2 * sizeof(cx<float>), h->st));
+  bool use_net = h->net != nullptr;
+  int nsteps = 0, nbad = 0;
+  // z = net(rs); a z nearly orthogonal to rs (cos < 1e-2) or not finite is
+  // replaced by rs for this step, and after 10 such steps the net is dropped
+  auto precondition = [&](double rrs) -> int {
+    bool ok = false;
+    if (use_net) {
+      TRY(ptc_apply<PS>(h, z, rs));
+      double rz, zz;
+      TRY(dot<PS>(h, rs, z, &rz));
+      TRY(L::norm(h, z, &zz));
+      ok = std::isfinite(rz) && std::isfinite(zz) && zz > 0.0 &&
+           rz > 1e-2 * std::sqrt(rrs * zz);
+      if (ok) ++nsteps;
+      else if (++nbad > 10) use_net = false;
+    }
+    if (!ok) TRY(L::copy(h, z, rs));
+    return DWFGPU_OK;
+  };
+  TRY(precondition(rr));
+  TRY(L::copy(h, ps, z));
+  double rr_upd = rr;
+  int itr = 0, nrel = 0, status = DWFGPU_OK, since = 0;
+  const int max_itr = arg->max_num_iter - 1;
+  const int stall = 400;
+  bool converged = rr <= stp;
+  while (!converged && itr < max_itr) {
+    ++itr;
+    ++since;
+    double d, rp;
+    TRY(L::mdagm(h, mass, qs, ps, &d));
+    if (d == 0.0) {
+      status = DWFGPU_BREAKDOWN;
+      break;
+    }
+    TRY(dot<PS>(h, rs, ps, &rp));
+    const double a = rp / d;  // exact line minimum along p
+    hipLaunchKernelGGL((dwfk::k_cg_update<PS, P32>), blas_grid(), blas_block(), 0, h->st,
+                       fld<P32>(xs), fld<PS>(rs), cfld<PS>(ps), cfld<PS>(qs), a, h->g, h->partial);
+    HIPCHK(hipGetLastError());
+    double rrn;
+    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rrn));
+    bool restart = false;
+    if (rrn <= stp || rrn < delta * delta * rr_upd || (use_net && since > stall)) {
+      if (use_net && since > stall && !(rrn < delta * delta * rr_upd)) use_net = false;
+      hipLaunchKernelGGL((dwfk::k_accumulate<P32>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P64>(sol), fld<P32>(xs), h->g);
+      HIPCHK(hipGetLastError());
+      TRY(H::mdagm(h, mass, ax, sol, nullptr));
+      TRY(H::axpy_to(h, r, ax, src, -1.0, true));
+      TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rrn));
+      hipLaunchKernelGGL((dwfk::k_convert<PS, P64>), blas_grid(), blas_block(), 0, h->st,
+                         fld<PS>(rs), cfld<P64>(r), h->g);
+      HIPCHK(hipGetLastError());
+      rr_upd = rrn;
+      ++nrel;
+      since = 0;
+      restart = true;
+      if (rrn <= stp) {
+        rr = rrn;
+        converged = true;
+        break;
+      }
+    }
+    TRY(precondition(rrn));
+    double beta = 0.0;
+    if (!restart) {
+      double zq;
+      TRY(dot<PS>(h, z, qs, &zq));
+      beta = -zq / d;
+    }
+    TRY(L::axpby(h, ps, z, 1.0, beta));  // p = z + beta p
+    rr = rrn;
+  }
+  if (!converged && status == DWFGPU_OK) {
+    hipLaunchKernelGGL((dwfk::k_accumulate<P32>), blas_grid(), blas_block(), 0, h->st,
+                       fld<P64>(sol), fld<P32>(xs), h->g);
+    HIPCHK(hipGetLastError());
+    TRY(H::mdagm(h, mass, ax, sol, nullptr));
+    TRY(H::axpy_to(h, r, ax, src, -1.0, true));
+    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
+    status = DWFGPU_MAXITER;
+  }
+  if (true_res) *true_res = std::sqrt(rr / sn);
+  if (iters) *iters = itr + 1;
+  if (nrel_out) *nrel_out = nrel;
+  if (net_steps) *net_steps = nsteps;
+  return status;
+}
+
+}  // namespace
 
 // ===========================================================================
 // Many right-hand sides (DWF): batched operator and the batched
@@ -1796,6 +2292,173 @@ int dwfgpu_mrhs_shape(dwfgpu_t h, int* ns, int* nr) {
   *ns = h->mr_last_ns;
   *nr = h->mr_last_nr;
   return DWFGPU_OK;
+}
+
+size_t dwfgpu_ptc_num_weights(dwfgpu_t h) {
+  if (!h) return 0;
+  const size_t ls = h->ls, hid = 16;
+  return (size_t)ptck::NDIR * 16 * (hid * ls + hid * hid + ls * hid);
+}
+
+int dwfgpu_ptc_init(dwfgpu_t h, unsigned long long seed, double scale) {
+  if (!h || h->prec != DWFGPU_F64) return DWFGPU_BAD_ARG;
+  TRY(ptc_ensure(h));
+  ptc_init_weights(h->net, seed, scale);
+  return ptc_upload(h);
+}
+
+int dwfgpu_ptc_set_weights(dwfgpu_t h, const float* w, size_t n) {
+  if (!h || !w || !h->net || n != dwfgpu_ptc_num_weights(h)) return DWFGPU_BAD_ARG;
+  size_t q = 0;
+  for (int k = 0; k < 3; ++k)
+    for (auto& x : h->net->W[k]) {
+      x = std::complex<float>(w[2 * q], w[2 * q + 1]);
+      ++q;
+    }
+  return ptc_upload(h);
+}
+
+int dwfgpu_ptc_get_weights(dwfgpu_t h, float* w, size_t n) {
+  if (!h || !w || !h->net || n != dwfgpu_ptc_num_weights(h)) return DWFGPU_BAD_ARG;
+  size_t q = 0;
+  for (int k = 0; k < 3; ++k)
+    for (auto& x : h->net->W[k]) {
+      w[2 * q] = x.real();
+      w[2 * q + 1] = x.imag();
+      ++q;
+    }
+  return DWFGPU_OK;
+}
+
+int dwfgpu_ptc_apply(dwfgpu_t h, double* out, const double* in) {
+  if (!h || !out || !in || !h->net) return DWFGPU_BAD_ARG;
+  TRY(ensure_work<P32>(h, 6));
+  const DVec& a = h->f[DWFGPU_F32].work[4];
+  const DVec& b = h->f[DWFGPU_F32].work[5];
+  TRY(Ops<P32>::to_device(h, a, in));
+  TRY(ptc_apply<P32>(h, b, a));
+  return Ops<P32>::to_host(h, out, b);
+}
+
+int dwfgpu_ptc_loss_grad(dwfgpu_t h, const double* r, const double* e, double* loss,
+                         float* grads) {
+  if (!h || !r || !e || !h->net) return DWFGPU_BAD_ARG;
+  TRY(ensure_work<P32>(h, 6));
+  const DVec& a = h->f[DWFGPU_F32].work[4];
+  const DVec& b = h->f[DWFGPU_F32].work[5];
+  TRY(Ops<P32>::to_device(h, a, r));
+  TRY(Ops<P32>::to_device(h, b, e));
+  TRY(ptc_grad_pair(h, a, b, false, loss));
+  if (grads) {
+    size_t off = 0;
+    for (int k = 0; k < 3; ++k) {
+      HIPCHK(hipMemcpyAsync(grads + 2 * off, h->net->G[k], h->net->nw(k) * sizeof(cx<float>),
+                            hipMemcpyDeviceToHost, h->st));
+      off += h->net->nw(k);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
+}
+
+int dwfgpu_ptc_train(dwfgpu_t h, const dwfgpu_ptc_train_arg* a, double* loss_first,
+                     double* loss_last, int* pairs_used) {
+  if (!h || !a || !h->net || a->nsolves < 1 || a->pairs_per_solve < 0 || a->random_pairs < 0 ||
+      a->pairs_per_solve + a->random_pairs < 1 || a->stride < 1 || a->epochs < 1 ||
+      a->batch < 1 || !(a->lr > 0.0))
+    return DWFGPU_BAD_ARG;
+  const int P = a->pairs_per_solve + a->random_pairs;
+  std::vector<DVec> R(P), E(P);
+  DVec b64;
+  int st = alloc_vec<P64>(h, &b64);
+  for (int j = 0; j < P && !st; ++j) {
+    st = alloc_vec<P32>(h, &R[j]);
+    if (!st) st = alloc_vec<P32>(h, &E[j]);
+  }
+  std::vector<double> host(h->n);
+  double first = -1.0, last = 0.0;
+  int used = 0;
+  for (int sidx = 0; sidx < a->nsolves && !st; ++sidx) {
+    dwfsynth_gaussian(host.data(), h->n, a->seed + 7919ull * sidx);
+    st = Ops<P64>::to_device(h, b64, host.data());
+    int made = 0;
+    if (!st && a->pairs_per_solve > 0)
+      st = ptc_pairs(h, a->mass, b64, a->stride, a->pairs_per_solve, R, E, &made);
+    // pairs whose correction is a Gaussian vector e and residual r = A e:
+    // they carry the high modes the CG residuals lack
+    for (int j = 0; j < a->random_pairs && !st; ++j) {
+      dwfsynth_gaussian(host.data(), h->n, a->seed + 104729ull * (sidx * 977 + j + 1));
+      st = Ops<P32>::to_device(h, E[made], host.data());
+      if (!st) st = Ops<P32>::mdagm(h, a->mass, R[made], E[made], nullptr);
+      if (!st) ++made;
+    }
+    if (st) break;
+    // interleave the two kinds in the mini-batches
+    if (a->random_pairs > 0) {
+      std::vector<int> order(made);
+      for (int j = 0; j < made; ++j) order[j] = j;
+      std::mt19937_64 rng(a->seed + sidx);
+      std::shuffle(order.begin(), order.end(), rng);
+      std::vector<DVec> R2(made), E2(made);
+      for (int j = 0; j < made; ++j) {
+        R2[j] = R[order[j]];
+        E2[j] = E[order[j]];
+      }
+      for (int j = 0; j < made; ++j) {
+        R[j] = R2[j];
+        E[j] = E2[j];
+      }
+    }
+    used += made;
+    for (int ep = 0; ep < a->epochs && !st; ++ep) {
+      double sum = 0.0;
+      for (int j0 = 0; j0 < made && !st; j0 += a->batch) {
+        const int j1 = std::min(made, j0 + a->batch);
+        for (int j = j0; j < j1 && !st; ++j) {
+          double l = 0.0;
+          st = ptc_grad_pair(h, R[j], E[j], j > j0, &l);
+          sum += l;
+          if (first < 0.0) first = l;
+        }
+        if (!st) st = ptc_adam(h, j1 - j0, a->lr);
+      }
+      if (made > 0) last = sum / made;
+    }
+  }
+  (void)hipStreamSynchronize(h->st);
+  free_vec(&b64);
+  for (int j = 0; j < P; ++j) {
+    free_vec(&R[j]);
+    free_vec(&E[j]);
+  }
+  if (loss_first) *loss_first = first;
+  if (loss_last) *loss_last = last;
+  if (pairs_used) *pairs_used = used;
+  return st;
+}
+
+int dwfgpu_invcg_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, double delta,
+                         dwfgpu_vec_t sol, dwfgpu_vec_t src, double* true_res, int* iters,
+                         int* reliable_updates, int* net_steps) {
+  if (!h || !arg || !sol || !src || sol->h != h || src->h != h) return DWFGPU_BAD_ARG;
+  if (h->prec != DWFGPU_F64 || arg->max_num_iter < 1 || !(delta > 0.0 && delta < 1.0)) {
+    set_err("dwfgpu_invcg_ptc_dev: FP64 handle, max_num_iter >= 1 and 0 < delta < 1 needed");
+    return DWFGPU_BAD_ARG;
+  }
+  switch (sloppy) {
+    case DWFGPU_F32:
+      return invfcg_ptc<P32>(h, arg, delta, sol->v, src->v, true_res, iters, reliable_updates,
+                             net_steps);
+    case DWFGPU_I16:
+      return invfcg_ptc<PI16>(h, arg, delta, sol->v, src->v, true_res, iters, reliable_updates,
+                              net_steps);
+    case DWFGPU_BF16:
+      return invfcg_ptc<PBF16>(h, arg, delta, sol->v, src->v, true_res, iters, reliable_updates,
+                               net_steps);
+    default:
+      set_err("dwfgpu_invcg_ptc_dev: sloppy precision must be F32, BF16 or I16");
+      return DWFGPU_BAD_ARG;
+  }
 }
 
 }  // extern "C"

@@ -1,0 +1,410 @@
+// ptc_kernels.hpp -- gfx950 kernels of the gauge-equivariant parallel-
+// transport network used as a preconditioner of the CG on MatPcDagMatPc.
+//
+// Network (no CPS counterpart): three layers Ls -> H -> H -> Ls channels,
+//   out_o(x) = sum_{d=0..8} sum_i W[d][o][i] (T_d phi_i)(x)
+// with T_0 = 1, T_{+mu} phi(x) = U_mu(x) phi(x+mu), T_{-mu} phi(x) =
+// U_mu(x-mu)^dag phi(x-mu) (d = 1..4: +x..+t, 5..8: -x..-t) and W[d][o][i] a
+// complex 4x4 spin matrix.  Colour is only ever moved by links, so the map is
+// gauge covariant.
+//
+// Activations: the full 4D lattice, site S = parity*vh + i (i the tiled
+// half-lattice index of the operator), per site C channels x 4 spins x 3
+// colours of bf16 complex numbers (one uint32: re low, im high).
+//
+// Each layer is a dense contraction, run on the MFMA units:
+//   C[M' x N] = sum_d A_d[M' x K'] B_d[K' x N]
+// with M' = 8*Cout (re/im x channel x spin), K' = 8*Cin, N = 3 colours x 32
+// sites of a tile, A_d = [[Re W_d, -Im W_d], [Im W_d, Re W_d]] in bf16 and
+// B_d the transported neighbour fields (bf16, FP32 accumulation).  The
+// weight gradient is the contraction the other way round,
+//   P_d[M' x K'] = sum_{sites, colours} X[M' x N] Y_d[K' x N]^T
+// with X the output gradient and Y_d the transported input.
+// MFMA: v_mfma_f32_16x16x16_bf16 (lane l holds A[l%16][4(l/16)..+3],
+// B[4(l/16)..+3][l%16] and C[4(l/16)..+3][l%16]).
+#pragma once
+#include "dwf_kernels.hpp"
+
+namespace ptck {
+
+using dwfk::cx;
+using dwfk::Geom;
+
+constexpr int NDIR = 9;
+constexpr int NT = 32;          // sites per tile
+constexpr int NCOL = 3 * NT;    // GEMM columns per tile
+constexpr int NTHR = 256;       // threads (4 waves) per work-group
+constexpr int PADK = 8;         // LDS row padding (bf16 elements)
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float x) { return (uint16_t)dwfk::bf16_rne(x); }
+__device__ __forceinline__ float bf2f(uint32_t b16) { return __uint_as_float(b16 << 16); }
+__device__ __forceinline__ uint32_t pack_c(cx<float> v) {
+  return (uint32_t)f2bf(v.re) | ((uint32_t)f2bf(v.im) << 16);
+}
+__device__ __forceinline__ cx<float> unpack_c(uint32_t w) {
+  return {__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)};
+}
+
+// 4D parity held by slice s of the CG half vector: DWF (s+1)%2, Mobius 1
+__host__ __device__ __forceinline__ int slice_parity(int action, int s) {
+  return action == 0 ? (s + 1) & 1 : 1;
+}
+
+// CG half vector (policy P, times `scale`) -> activations with C = ls
+// channels; the other parity of every channel is cleared
+template <class P>
+__global__ void __launch_bounds__(256)
+k_ptc_embed(uint32_t* __restrict__ act, dwfk::CFld<P> in, Geom g, int action, float scale) {
+  const int C = g.ls;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < (size_t)g.vh * g.ls;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(k / g.vh), i = (int)(k - (size_t)s * g.vh);
+    const int p = slice_parity(action, s);
+    cx<typename P::R> v[12];
+    dwfk::ld_spinor<P>(in.d + (size_t)s * dwfk::nch<P>() * g.vh, in.sc + (size_t)s * g.vh, g.vh,
+                       i, v);
+    uint32_t* dst = act + ((size_t)(p * g.vh + i) * C + s) * 12;
+    uint32_t* zer = act + ((size_t)((1 - p) * g.vh + i) * C + s) * 12;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      dst[c] = pack_c(cx<float>{(float)v[c].re * scale, (float)v[c].im * scale});
+      zer[c] = 0u;
+    }
+  }
+}
+
+// activations (C = ls) -> CG half vector (policy P, times `scale`)
+template <class P>
+__global__ void __launch_bounds__(256)
+k_ptc_extract(dwfk::Fld<P> out, const uint32_t* __restrict__ act, Geom g, int action,
+              float scale) {
+  typedef typename P::R R;
+  const int C = g.ls;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < (size_t)g.vh * g.ls;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(k / g.vh), i = (int)(k - (size_t)s * g.vh);
+    const int p = slice_parity(action, s);
+    const uint32_t* src = act + ((size_t)(p * g.vh + i) * C + s) * 12;
+    cx<R> v[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      const cx<float> w = unpack_c(src[c]);
+      v[c] = cx<R>{(R)(w.re * scale), (R)(w.im * scale)};
+    }
+    dwfk::st_site<P>(out.d + (size_t)s * dwfk::nch<P>() * g.vh, out.sc + (size_t)s * g.vh, g.vh,
+                     i, v);
+  }
+}
+
+// Training loss of one pair: diff = extract(a3) - tscale * target (FP32 CG
+// vector), partial sums of |diff|^2, and the output gradient diff * gscale
+// embedded as activations (gscale = 1/|tscale * target|^2: the derivative of
+// the normalised loss with respect to conj(output)).
+__global__ void __launch_bounds__(256)
+k_ptc_loss(uint32_t* __restrict__ gact, const uint32_t* __restrict__ a3,
+           dwfk::CFld<dwfk::P32> target, float tscale, Geom g, int action, float gscale,
+           double* __restrict__ partial) {
+  __shared__ double sh[16];
+  const int C = g.ls;
+  double acc = 0.0;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < (size_t)g.vh * g.ls;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const int s = (int)(k / g.vh), i = (int)(k - (size_t)s * g.vh);
+    const int p = slice_parity(action, s);
+    const size_t base = ((size_t)(p * g.vh + i) * C + s) * 12;
+    cx<float> t[12];
+    dwfk::ld_spinor<dwfk::P32>(target.d + (size_t)s * 6 * g.vh, nullptr, g.vh, i, t);
+    uint32_t* zer = gact + ((size_t)((1 - p) * g.vh + i) * C + s) * 12;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) {
+      const cx<float> o = unpack_c(a3[base + c]);
+      const cx<float> d{o.re - tscale * t[c].re, o.im - tscale * t[c].im};
+      acc += (double)d.re * d.re + (double)d.im * d.im;
+      gact[base + c] = pack_c(cx<float>{d.re * gscale, d.im * gscale});
+      zer[c] = 0u;
+    }
+  }
+  const double tot = dwfk::block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// transported colour vector: d = 0 identity, 1..4 U phi, 5..8 U^dag phi
+__device__ __forceinline__ void transport3(const cx<float>* u, int d, const cx<float>* f,
+                                           cx<float>* v) {
+  if (d == 0) {
+    v[0] = f[0];
+    v[1] = f[1];
+    v[2] = f[2];
+  } else if (d <= 4) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      v[a] = dwfk::row_dot(u[a * 3 + 0], u[a * 3 + 1], u[a * 3 + 2], f[0], f[1], f[2]);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+      v[a] = dwfk::row_dotc(u[0 * 3 + a], u[1 * 3 + a], u[2 * 3 + a], f[0], f[1], f[2]);
+  }
+}
+
+// site of the activation neighbour of tile site n along d (S index)
+__device__ __forceinline__ size_t nbr_site(const int* lnb, int d, int n, int par, int i0,
+                                           const Geom& g) {
+  if (d == 0) return (size_t)par * g.vh + i0 + n;
+  return (size_t)(1 - par) * g.vh + lnb[(d - 1) * NT + n];
+}
+
+// Fill the transported field of direction d for the tile into LDS, as bf16,
+// either [col][K'] (KMAJOR = false, the B operand of the forward GEMM) or
+// [K'][col] (KMAJOR = true, the Y operand of the weight gradient).
+// k' = part*(CIN*4) + i*4 + spin, col = n*3 + colour.
+template <int CIN, bool KMAJOR>
+__device__ __forceinline__ void build_transported(uint16_t* sB, const uint32_t* __restrict__ in,
+                                                  const cx<float>* lu, const int* lnb, int d,
+                                                  int par, int i0, const Geom& g) {
+  constexpr int KP = 8 * CIN;
+  for (int w = threadIdx.x; w < NT * CIN * 4; w += NTHR) {
+    const int n = w / (CIN * 4), r = w - n * (CIN * 4);
+    const int i = r >> 2, sp = r & 3;
+    const size_t y = nbr_site(lnb, d, n, par, i0, g);
+    const uint32_t* src = in + (y * CIN + i) * 12 + sp * 3;
+    cx<float> f[3], v[3];
+    f[0] = unpack_c(src[0]);
+    f[1] = unpack_c(src[1]);
+    f[2] = unpack_c(src[2]);
+    transport3(d == 0 ? nullptr : lu + ((d - 1) * NT + n) * 9, d, f, v);
+    const int kre = i * 4 + sp, kim = CIN * 4 + i * 4 + sp;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int col = n * 3 + c;
+      if (KMAJOR) {
+        sB[kre * (NCOL + PADK) + col] = f2bf(v[c].re);
+        sB[kim * (NCOL + PADK) + col] = f2bf(v[c].im);
+      } else {
+        sB[col * (KP + PADK) + kre] = f2bf(v[c].re);
+        sB[col * (KP + PADK) + kim] = f2bf(v[c].im);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ bf16x4 ld4(const uint16_t* p) {
+  const uint2 w = *reinterpret_cast<const uint2*>(p);
+  bf16x4 r;
+  r[0] = (short)(w.x & 0xffff);
+  r[1] = (short)(w.x >> 16);
+  r[2] = (short)(w.y & 0xffff);
+  r[3] = (short)(w.y >> 16);
+  return r;
+}
+
+// LDS bytes of the forward kernel
+template <int CIN, int COUT>
+constexpr size_t fwd_lds() {
+  constexpr size_t a = (size_t)(8 * COUT) * (8 * CIN + PADK) * 2;
+  constexpr size_t b = (size_t)NCOL * (8 * CIN + PADK) * 2;
+  constexpr size_t c = (size_t)(8 * COUT) * NCOL * 4;
+  constexpr size_t links = (size_t)8 * NT * 9 * 8 + 8 * NT * 4;
+  return (a + b > c ? a + b : c) + links;
+}
+
+// One layer on one tile of NT sites per work-group:
+// out[S][o][spin][colour] = sum_d W_d T_d in.  A: [9][M'][K'] bf16.
+// grid.x = 2 * vh / NT tiles (first all even-parity tiles, then odd).
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(NTHR)
+k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
+          const uint16_t* __restrict__ A, const typename dwfk::P32::S* __restrict__ U, Geom g) {
+  constexpr int MP = 8 * COUT, KP = 8 * CIN;
+  constexpr int MB = MP / 16, NB = NCOL / 16, KB = KP / 16;
+  constexpr int NBLK = MB * NB, PER = (NBLK + 3) / 4;
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint16_t* sA = reinterpret_cast<uint16_t*>(smem);
+  uint16_t* sB = sA + MP * (KP + PADK);
+  float* sC = reinterpret_cast<float*>(smem);  // after the last GEMM
+  constexpr size_t big = fwd_lds<CIN, COUT>() - ((size_t)8 * NT * 9 * 8 + 8 * NT * 4);
+  cx<float>* lu = reinterpret_cast<cx<float>*>(smem + big);
+  int* lnb = reinterpret_cast<int*>(smem + big + (size_t)8 * NT * 9 * 8);
+
+  const int tiles_per_par = g.vh / NT;
+  const int par = blockIdx.x / tiles_per_par;
+  const int i0 = (blockIdx.x - par * tiles_per_par) * NT;
+  dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par);
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  f32x4 acc[PER];
+#pragma unroll
+  for (int b = 0; b < PER; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int d = 0; d < NDIR; ++d) {
+    __syncthreads();  // links staged / previous direction's operands consumed
+    {
+      const uint4* src = reinterpret_cast<const uint4*>(A + (size_t)d * MP * KP);
+      for (int w = threadIdx.x; w < MP * KP / 8; w += NTHR) {
+        const int m = w / (KP / 8), k8 = w - m * (KP / 8);
+        *reinterpret_cast<uint4*>(sA + m * (KP + PADK) + k8 * 8) = src[w];
+      }
+    }
+    build_transported<CIN, false>(sB, in, lu, lnb, d, par, i0, g);
+    __syncthreads();
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const int k0 = kb * 16 + 4 * (lane >> 4);
+#pragma unroll
+      for (int b = 0; b < PER; ++b) {
+        const int blk = wave + 4 * b;
+        if (blk < NBLK) {
+          const int mb = blk / NB, nb = blk - mb * NB;
+          const bf16x4 af = ld4(sA + (mb * 16 + (lane & 15)) * (KP + PADK) + k0);
+          const bf16x4 bfr = ld4(sB + (nb * 16 + (lane & 15)) * (KP + PADK) + k0);
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[b], 0, 0, 0);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // C (lane: rows 4(l/16)..+3 of column l%16) -> LDS [M'][NCOL]
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int blk = wave + 4 * b;
+    if (blk < NBLK) {
+      const int mb = blk / NB, nb = blk - mb * NB;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        sC[(mb * 16 + 4 * (lane >> 4) + r) * NCOL + nb * 16 + (lane & 15)] = acc[b][r];
+    }
+  }
+  __syncthreads();
+  // out[S][o][spin][colour]: re row o*4+spin, im row COUT*4 + o*4 + spin
+  for (int w = threadIdx.x; w < NT * COUT * 12; w += NTHR) {
+    const int n = w / (COUT * 12), r = w - n * (COUT * 12);
+    const int os = r / 3, c = r - os * 3;
+    const float re = sC[os * NCOL + n * 3 + c], im = sC[(COUT * 4 + os) * NCOL + n * 3 + c];
+    out[((size_t)par * g.vh + i0 + n) * (COUT * 12) + r] = pack_c(cx<float>{re, im});
+  }
+}
+
+template <int CIN, int COUT>
+constexpr size_t gw_lds() {
+  constexpr size_t x = (size_t)(8 * COUT) * (NCOL + PADK) * 2;
+  constexpr size_t y = (size_t)(8 * CIN) * (NCOL + PADK) * 2;
+  return x + y + (size_t)8 * NT * 9 * 8 + 8 * NT * 4;
+}
+
+// Weight gradient partials: for direction d = blockIdx.y and the tiles
+// t = blockIdx.x, blockIdx.x + gridDim.x, ...:
+// part[d][blockIdx.x][M'][K'] = sum X[M'][cols] Y_d[K'][cols]^T
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(NTHR)
+k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
+         const uint32_t* __restrict__ in, const typename dwfk::P32::S* __restrict__ U, Geom g) {
+  constexpr int MP = 8 * COUT, KP = 8 * CIN;
+  constexpr int MB = MP / 16, KB2 = KP / 16, CB = NCOL / 16;
+  constexpr int NBLK = MB * KB2, PER = (NBLK + 3) / 4;
+  extern __shared__ __align__(16) unsigned char smem[];
+  uint16_t* sX = reinterpret_cast<uint16_t*>(smem);
+  uint16_t* sY = sX + MP * (NCOL + PADK);
+  cx<float>* lu = reinterpret_cast<cx<float>*>(smem + (size_t)(MP + KP) * (NCOL + PADK) * 2);
+  int* lnb = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(lu) + (size_t)8 * NT * 9 * 8);
+  const int d = blockIdx.y;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int tiles_per_par = g.vh / NT, ntiles = 2 * tiles_per_par;
+  f32x4 acc[PER];
+#pragma unroll
+  for (int b = 0; b < PER; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const int par = t / tiles_per_par;
+    const int i0 = (t - par * tiles_per_par) * NT;
+    __syncthreads();  // previous tile's operands consumed
+    dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par);
+    // X: the output gradient of the tile, [M'][col]
+    for (int w = threadIdx.x; w < NT * COUT * 12; w += NTHR) {
+      const int n = w / (COUT * 12), r = w - n * (COUT * 12);
+      const int os = r / 3, c = r - os * 3;
+      const cx<float> v = unpack_c(gout[((size_t)par * g.vh + i0 + n) * (COUT * 12) + r]);
+      sX[os * (NCOL + PADK) + n * 3 + c] = f2bf(v.re);
+      sX[(COUT * 4 + os) * (NCOL + PADK) + n * 3 + c] = f2bf(v.im);
+    }
+    __syncthreads();  // links staged
+    build_transported<CIN, true>(sY, in, lu, lnb, d, par, i0, g);
+    __syncthreads();
+#pragma unroll
+    for (int cb = 0; cb < CB; ++cb) {
+      const int c0 = cb * 16 + 4 * (lane >> 4);
+#pragma unroll
+      for (int b = 0; b < PER; ++b) {
+        const int blk = wave + 4 * b;
+        if (blk < NBLK) {
+          const int mb = blk / KB2, kb = blk - mb * KB2;
+          const bf16x4 xf = ld4(sX + (mb * 16 + (lane & 15)) * (NCOL + PADK) + c0);
+          const bf16x4 yf = ld4(sY + (kb * 16 + (lane & 15)) * (NCOL + PADK) + c0);
+          acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xf, yf, acc[b], 0, 0, 0);
+        }
+      }
+    }
+  }
+  float* dst = part + ((size_t)d * gridDim.x + blockIdx.x) * MP * KP;
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    const int blk = wave + 4 * b;
+    if (blk < NBLK) {
+      const int mb = blk / KB2, kb = blk - mb * KB2;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        dst[(mb * 16 + 4 * (lane >> 4) + r) * KP + kb * 16 + (lane & 15)] = acc[b][r];
+    }
+  }
+}
+
+// G[d][o][i][a][b] (complex, FP32) = 2 * (P_re,re + P_im,im, P_im,re - P_re,im)
+// summed over the nwg partials in a fixed order
+__global__ void __launch_bounds__(256)
+k_ptc_gw_reduce(cx<float>* __restrict__ G, const float* __restrict__ part, int nwg, int cin,
+                int cout, int accumulate) {
+  const int MP = 8 * cout, KP = 8 * cin;
+  const int n = NDIR * cout * 4 * cin * 4;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+    int r = w;
+    const int b = r % 4;
+    r /= 4;
+    const int i = r % cin;
+    r /= cin;
+    const int a = r % 4;
+    r /= 4;
+    const int o = r % cout;
+    const int d = r / cout;
+    const int mre = o * 4 + a, mim = cout * 4 + o * 4 + a;
+    const int kre = i * 4 + b, kim = cin * 4 + i * 4 + b;
+    double re = 0.0, im = 0.0;
+    for (int q = 0; q < nwg; ++q) {
+      const float* P = part + ((size_t)d * nwg + q) * MP * KP;
+      re += (double)P[mre * KP + kre] + (double)P[mim * KP + kim];
+      im += (double)P[mim * KP + kre] - (double)P[mre * KP + kim];
+    }
+    // G index [d][o][i][a][b]
+    cx<float>& gv = G[(((size_t)(d * cout + o) * cin + i) * 4 + a) * 4 + b];
+    const cx<float> old = accumulate ? gv : cx<float>{0.f, 0.f};
+    gv = cx<float>{old.re + (float)(2.0 * re), old.im + (float)(2.0 * im)};
+  }
+}
+
+// Re <x, y> partial sums over a half vector
+template <class P>
+__global__ void __launch_bounds__(dwfk::BLAS_THREADS)
+k_dot(dwfk::CFld<P> x, dwfk::CFld<P> y, Geom g, double* __restrict__ partial) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  DWFK_SITE_LOOP(k, g) {
+    cx<typename P::R> a[12], b[12];
+    dwfk::ld_site<P>(a, x, g, k);
+    dwfk::ld_site<P>(b, y, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) acc += (double)a[c].re * b[c].re + (double)a[c].im * b[c].im;
+  }
+  const double t = dwfk::block_sum(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+}  // namespace ptck

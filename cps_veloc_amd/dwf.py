@@ -41,6 +41,9 @@ EXPORTS = (
     "dwfgpu_invcg_mixed", "dwfgpu_invcg_mixed_dev",
     "dwfgpu_mvec_create", "dwfgpu_mvec_destroy", "dwfgpu_mvec_upload", "dwfgpu_mvec_download",
     "dwfgpu_matpcdagmatpc_mrhs_dev", "dwfgpu_invcg_mrhs_dev", "dwfgpu_mrhs_shape",
+    "dwfgpu_ptc_num_weights", "dwfgpu_ptc_init", "dwfgpu_ptc_set_weights",
+    "dwfgpu_ptc_get_weights", "dwfgpu_ptc_apply", "dwfgpu_ptc_loss_grad", "dwfgpu_ptc_train",
+    "dwfgpu_invcg_ptc_dev",
     "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
 )
 # every entry point of include/dwf_synth.h
@@ -52,6 +55,13 @@ class DwfGpuError(RuntimeError):
         msg = lib().dwfgpu_last_error().decode(errors="replace")
         super().__init__(f"{where} failed with status {code}: {msg}")
         self.code = code
+
+
+class PtcTrainArgC(C.Structure):
+    _fields_ = [("mass", C.c_double), ("nsolves", C.c_int), ("pairs_per_solve", C.c_int),
+                ("random_pairs", C.c_int), ("stride", C.c_int), ("epochs", C.c_int),
+                ("batch", C.c_int),
+                ("lr", C.c_double), ("seed", C.c_ulonglong)]
 
 
 class CgArgC(C.Structure):
@@ -116,6 +126,19 @@ def lib():
         L.dwfgpu_invcg_mrhs_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, C.c_int,
                                             _dp, H, H, _dp, _ip, _ip, _ip]
         L.dwfgpu_mrhs_shape.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.dwfgpu_ptc_num_weights.argtypes = [H]
+        L.dwfgpu_ptc_num_weights.restype = C.c_size_t
+        L.dwfgpu_ptc_init.argtypes = [H, C.c_ulonglong, C.c_double]
+        L.dwfgpu_ptc_set_weights.argtypes = [H, _fp, C.c_size_t]
+        L.dwfgpu_ptc_get_weights.argtypes = [H, _fp, C.c_size_t]
+        L.dwfgpu_ptc_apply.argtypes = [H, _dp, _dp]
+        L.dwfgpu_ptc_loss_grad.argtypes = [H, _dp, _dp, C.POINTER(C.c_double), _fp]
+        L.dwfgpu_ptc_train.argtypes = [H, C.POINTER(PtcTrainArgC), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.dwfgpu_invcg_ptc_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, H, H,
+                                           C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.dwfgpu_time_matpcdagmatpc.argtypes = [H, C.c_int, C.c_double, H, H, C.c_int,
                                                 C.POINTER(C.c_double), _dp, _dp,
                                                 C.POINTER(C.c_int)]
@@ -321,6 +344,55 @@ class _DiracOpGpu:
                                          1 if eager else 0, _arr(masses), sol.h, src.h, tr, it,
                                          nr, stv), "InvCgMrhs_dev")
         return it, tr, stv, nr
+
+    # --- learned preconditioner -------------------------------------------
+    def ptc_init(self, seed=1, scale=0.02):
+        _chk(lib().dwfgpu_ptc_init(self.h, seed, scale), "ptc_init")
+
+    def ptc_num_weights(self):
+        return int(lib().dwfgpu_ptc_num_weights(self.h))
+
+    def ptc_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.complex64)
+        f = w.view(np.float32)
+        _chk(lib().dwfgpu_ptc_set_weights(self.h, f, w.size), "ptc_set_weights")
+
+    def ptc_get_weights(self):
+        f = np.zeros(2 * self.ptc_num_weights(), dtype=np.float32)
+        _chk(lib().dwfgpu_ptc_get_weights(self.h, f, f.size // 2), "ptc_get_weights")
+        return f.view(np.complex64)
+
+    def ptc_apply(self, r):
+        out = np.zeros(self.n)
+        _chk(lib().dwfgpu_ptc_apply(self.h, out, _arr(r)), "ptc_apply")
+        return out
+
+    def ptc_loss_grad(self, r, e):
+        loss = C.c_double()
+        g = np.zeros(2 * self.ptc_num_weights(), dtype=np.float32)
+        _chk(lib().dwfgpu_ptc_loss_grad(self.h, _arr(r), _arr(e), C.byref(loss), g),
+             "ptc_loss_grad")
+        return loss.value, g.view(np.complex64)
+
+    def ptc_train(self, mass, nsolves=2, pairs_per_solve=16, random_pairs=16, stride=4,
+                  epochs=4, batch=4, lr=1e-3, seed=7):
+        a = PtcTrainArgC(mass, nsolves, pairs_per_solve, random_pairs, stride, epochs, batch, lr,
+                         seed)
+        l0, l1, n = C.c_double(), C.c_double(), C.c_int()
+        _chk(lib().dwfgpu_ptc_train(self.h, C.byref(a), C.byref(l0), C.byref(l1), C.byref(n)),
+             "ptc_train")
+        return l0.value, l1.value, n.value
+
+    def InvCgPtc_dev(self, sol: DeviceVector, src: DeviceVector, cg_arg: CgArg, sloppy="f32",
+                     delta=0.1):
+        """Flexible CG with the network as preconditioner; returns
+        (iters, true_res, status, reliable_updates, net_steps)."""
+        tr, it, nr, ns = C.c_double(), C.c_int(), C.c_int(), C.c_int()
+        a = cg_arg.c()
+        st = _chk(lib().dwfgpu_invcg_ptc_dev(self.h, C.byref(a), _PREC[sloppy], delta, sol.h,
+                                             src.h, C.byref(tr), C.byref(it), C.byref(nr),
+                                             C.byref(ns)), "InvCgPtc_dev", allow=(OK, MAXITER))
+        return it.value, tr.value, st, nr.value, ns.value
 
     def mrhs_shape(self):
         ns, nr = C.c_int(), C.c_int()

@@ -162,6 +162,50 @@ int dwfgpu_invcg_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, doub
 /* the launch shape (sites, RHS per work-group) the last batched call used */
 int dwfgpu_mrhs_shape(dwfgpu_t h, int *ns, int *nr);
 
+/* ---- learned preconditioner (no CPS counterpart) ------------------------ */
+/* A gauge-equivariant parallel-transport network z = |r| N(r/|r|) acting on
+ * CG vectors: three layers Ls -> 16 -> 16 -> Ls channels (Ls in {4,8,12,16}),
+ *   out_o(x) = sum_{d=0..8} sum_i W[d][o][i] (T_d in_i)(x),
+ * T_0 = 1, T_{+mu} f(x) = U_mu(x) f(x+mu), T_{-mu} f(x) = U_mu(x-mu)^dag f(x-mu)
+ * (d = 1..4: +x..+t, 5..8: -x..-t), W[d][o][i] a complex 4x4 spin matrix.
+ * Weights: complex numbers as (re, im) FP32 pairs, layers concatenated, each
+ * [9][cout][cin][4][4]; dwfgpu_ptc_num_weights gives their count.  BF16 MFMA
+ * kernels with FP32 accumulation run the layers and their gradients; the
+ * handle's API precision must be DWFGPU_F64.                                  */
+size_t dwfgpu_ptc_num_weights(dwfgpu_t h);
+/* create the network (once) and set random weights: scale * N(0,1) on every
+ * weight plus a unit diagonal on the local spin term, so that the initial
+ * network is close to z = 0.5 r                                               */
+int dwfgpu_ptc_init(dwfgpu_t h, unsigned long long seed, double scale);
+int dwfgpu_ptc_set_weights(dwfgpu_t h, const float *w, size_t n);
+int dwfgpu_ptc_get_weights(dwfgpu_t h, float *w, size_t n);
+/* z = N(r) on host CPS half vectors (computed in FP32 / BF16) */
+int dwfgpu_ptc_apply(dwfgpu_t h, double *out, const double *in);
+/* loss |N(r)/|r| - e/|r||^2 / |e/|r||^2 of one pair and its gradient
+ * dL/dRe W + i dL/dIm W (same layout as the weights)                         */
+int dwfgpu_ptc_loss_grad(dwfgpu_t h, const double *r, const double *e, double *loss,
+                         float *grads);
+/* Training on the GPU: for each of nsolves Gaussian sources (seeded), solve
+ * A x* = b, run FP32 CG from 0 keeping x_k every `stride` iterations
+ * (pairs_per_solve of them), form pairs (r_k = A e_k, e_k = x* - x_k), add
+ * random_pairs pairs (r = A e, e Gaussian), and run `epochs` Adam passes over
+ * them in mini-batches of `batch`.                                           */
+typedef struct {
+  double mass;
+  int nsolves, pairs_per_solve, random_pairs, stride, epochs, batch;
+  double lr;
+  unsigned long long seed;
+} dwfgpu_ptc_train_arg;
+int dwfgpu_ptc_train(dwfgpu_t h, const dwfgpu_ptc_train_arg *a, double *loss_first,
+                     double *loss_last, int *pairs_used);
+/* InvCg's solve (same stop rule on the FP64 true residual) by flexible CG with
+ * the network as preconditioner, `sloppy` vectors and FP64 reliable updates;
+ * plain CG takes over when the network stops helping (*net_steps counts the
+ * steps that used it).  Without a network it is plain reliable-update CG.   */
+int dwfgpu_invcg_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, double delta,
+                         dwfgpu_vec_t sol, dwfgpu_vec_t src, double *true_res, int *iters,
+                         int *reliable_updates, int *net_steps);
+
 /* ---- instrumentation (the role of CPS's print_flops / CGflops) --------- */
 /* Times `reps` MatPcDagMatPc applications on the handle's stream with HIP
  * events around each of its kernel launches.  precision: the handle's API

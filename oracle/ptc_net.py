@@ -151,3 +151,25 @@ def apply_net(Ws, half, U, dims, ls, action):
     nr = np.linalg.norm(half)
     f = embed(np.asarray(half) / nr, dims, ls, action)
     return nr * extract(forward(Ws, f, U)[-1], dims, ls, action)
+
+
+def loss_and_grads_cg(Ws, r_half, e_half, U, dims, ls, action):
+    """Training loss of one (residual, correction) pair of CG half vectors:
+    with r^ = r/|r| and e^ = e/|r|,
+        L = |extract(net(embed(r^))) - e^|^2 / |e^|^2,
+    and dL/dW per layer (the gradient flows only through the parity that
+    extract keeps)."""
+    nr = np.linalg.norm(r_half)
+    rh = np.asarray(r_half) / nr
+    eh = np.asarray(e_half) / nr
+    acts = forward(Ws, embed(rh, dims, ls, action), U)
+    out = extract(acts[-1], dims, ls, action)
+    diff = out - eh
+    nt = float(np.dot(eh, eh))
+    L = float(np.dot(diff, diff)) / nt
+    g = embed(diff / nt, dims, ls, action)
+    grads = [None] * len(Ws)
+    for k in range(len(Ws) - 1, -1, -1):
+        grads[k] = weight_grad(g, acts[k], U)
+        g = layer_adjoint(Ws[k], g, U)
+    return L, grads

@@ -110,3 +110,26 @@ def test_embed_extract(action):
     assert np.array_equal(pn.extract(f, DIMS, ls, action), h)
     # the other parity of every slice stays empty
     assert np.count_nonzero(f) == np.count_nonzero(h.view(np.complex128))
+
+
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
+def test_cg_loss_gradients_finite_difference(action):
+    ls = 2
+    rng = np.random.default_rng(6)
+    U = pn.links_with_bc(lat.random_gauge(DIMS, 0.4, 9), DIMS)
+    Ws = rand_weights(rng, [ls, 2, 2, ls])
+    vh = int(np.prod(DIMS)) // 2
+    r = rng.standard_normal(ls * vh * 24)
+    e = rng.standard_normal(ls * vh * 24)
+    L0, grads = pn.loss_and_grads_cg(Ws, r, e, U, DIMS, ls, action)
+    eps = 1e-6
+    for k in range(3):
+        D = cfield(rng, Ws[k].shape)
+        Wp = [w.copy() for w in Ws]
+        Wm = [w.copy() for w in Ws]
+        Wp[k] = Wp[k] + eps * D
+        Wm[k] = Wm[k] - eps * D
+        fd = (pn.loss_and_grads_cg(Wp, r, e, U, DIMS, ls, action)[0] -
+              pn.loss_and_grads_cg(Wm, r, e, U, DIMS, ls, action)[0]) / (2 * eps)
+        an = np.real(np.vdot(grads[k], D))
+        assert abs(fd - an) < 1e-6 * max(1.0, abs(an)), (k, fd, an)
