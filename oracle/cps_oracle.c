@@ -880,3 +880,141 @@ int orc_matinv(double *out, double *in, const double *u, double mass, double sto
   free(save);
   return iter;
 }
+
+/* ---- (f3): DiracOp::MInvCG, multi-shift CG (d_op_base/noarch/minvcg.C:47-355)
+ * Solves (A + shift[s]) psi_s = chi for every shift, A = MatPcDagMatPc, with
+ * one A application per iteration; shift[0] is the base system.  type 0
+ * (SINGLE): psi[0] accumulates sum_s alpha[s] psi_s; type 1 (MULTI): psi_s
+ * separately (zeroed first).  rsdcg[s] holds the relative stop residual on
+ * entry and the final |r_s| on return (sqrt of c z_s^2, :336-345); iters[s]
+ * (may be NULL) the iteration at which shift s converged (0 if it did not).
+ * Returns CPS's k (:345-350): the loop count at exit.                        */
+int orc_minvcg(double *psi, const double *chi, double chi_norm, const double *shift,
+               int nshift, double *rsdcg, int type, const double *alpha, int max_num_iter,
+               int *iters, const double *u, double mass, const orc_params *p) {
+  const size_t n = orc_half_size(p);
+  double *r = (double *)malloc(n * sizeof(double));
+  double *ap = (double *)malloc(n * sizeof(double));
+  double *pv = (double *)malloc((size_t)nshift * n * sizeof(double));
+  int *convs = (int *)calloc((size_t)nshift, sizeof(int));
+  int *conv_at = (int *)calloc((size_t)nshift, sizeof(int));
+  double *bs = (double *)malloc(nshift * sizeof(double));
+  double *z[2];
+  z[0] = (double *)malloc(nshift * sizeof(double));
+  z[1] = (double *)malloc(nshift * sizeof(double));
+  double *rsd_sq = (double *)malloc(nshift * sizeof(double));
+  double *fin = (double *)malloc(nshift * sizeof(double));
+  work3 w = work_new(p);
+#define MDAGM(o, i, d)                                             \
+  do {                                                             \
+    if (p->action == 0)                                            \
+      orc_dwf_mdagm(o, u, i, d, mass, p, w.t1, w.t2);              \
+    else                                                           \
+      orc_mobius_mdagm(o, u, i, d, mass, p, w.t1, w.t2, w.t3);     \
+  } while (0)
+  /* d = <p0, (A + shift[0]) p0> (:147-156, :229-238) */
+#define SHIFTED_A(dout)                                            \
+  do {                                                             \
+    if (shift[0] > 0) {                                            \
+      MDAGM(ap, pv, NULL);                                         \
+      f_times_v1_plus_v2(ap, shift[0], pv, ap, n);                 \
+      double dd_ = 0.0;                                            \
+      for (size_t i_ = 0; i_ < n; ++i_) dd_ += pv[i_] * ap[i_];    \
+      dout = dd_;                                                  \
+    } else {                                                       \
+      MDAGM(ap, pv, &dout);                                        \
+    }                                                              \
+  } while (0)
+
+  const int nvec = type == 0 ? 1 : nshift;
+  if (type == 1) memset(psi, 0, (size_t)nshift * n * sizeof(double)); /* :118-121 */
+  memcpy(r, chi, n * sizeof(double));                                   /* :123-125 */
+  double cp = chi_norm;
+  for (int s = 0; s < nshift; ++s) memcpy(pv + (size_t)s * n, r, n * sizeof(double));
+  for (int s = 0; s < nshift; ++s) {
+    rsd_sq[s] = cp * rsdcg[s] * rsdcg[s]; /* :136-140 */
+  }
+  double d;
+  SHIFTED_A(d);
+  double b = -cp / d; /* :160 */
+  z[0][0] = 1.0;
+  z[1][0] = 1.0;
+  bs[0] = b;
+  int iz = 1;
+  for (int s = 1; s < nshift; ++s) { /* :166-172 */
+    z[1 - iz][s] = 1.0;
+    z[iz][s] = 1.0 / (1.0 - b * (shift[s] - shift[0]));
+    bs[s] = b * z[iz][s];
+  }
+  f_times_v1_plus_v2(r, b, ap, r, n); /* :175 */
+  double c = orc_norm_sq(r, n);
+  if (type == 0) {
+    for (int s = 0; s < nshift; ++s) f_times_v1_plus_v2(psi, -bs[s] * alpha[s], chi, psi, n);
+  } else {
+    for (int s = 0; s < nshift; ++s)
+      f_times_v1_plus_v2(psi + (size_t)s * n, -bs[s], chi, psi + (size_t)s * n, n);
+  }
+  int convp = c < rsd_sq[0] ? 1 : 0; /* :195 */
+  double a = c / cp;
+  double css = 0.0;
+  int k;
+  for (k = 1; k <= max_num_iter && !convp; k++) { /* :202 */
+    a = c / cp;
+    for (int s = 0; s < nshift; ++s) { /* :210-222 */
+      if (convs[s]) continue;
+      double *ps = pv + (size_t)s * n;
+      if (s == 0) {
+        f_times_v1_plus_v2(ps, a, ps, r, n);
+      } else {
+        const double as = a * (z[iz][s] * bs[s]) / (z[1 - iz][s] * b);
+        for (size_t i = 0; i < n; ++i) ps[i] *= as;
+        f_times_v1_plus_v2(ps, z[iz][s], r, ps, n);
+      }
+    }
+    cp = c;
+    SHIFTED_A(d);
+    const double bp = b;
+    b = -cp / d;
+    bs[0] = b;
+    iz = 1 - iz;
+    for (int s = 1; s < nshift; ++s) { /* :246-254 */
+      if (convs[s]) continue;
+      const double ztmp = z[1 - iz][s] * z[iz][s] * bp /
+                          (b * a * (z[iz][s] - z[1 - iz][s]) +
+                           z[iz][s] * bp * (1 - b * (shift[s] - shift[0])));
+      bs[s] = b * ztmp / z[1 - iz][s];
+      z[iz][s] = ztmp;
+    }
+    f_times_v1_plus_v2(r, b, ap, r, n); /* :257 */
+    c = orc_norm_sq(r, n);
+    for (int s = 0; s < nshift; ++s) { /* :266-285 */
+      if (convs[s]) continue;
+      double *dst = type == 0 ? psi : psi + (size_t)s * n;
+      const double f = type == 0 ? -bs[s] * alpha[s] : -bs[s];
+      f_times_v1_plus_v2(dst, f, pv + (size_t)s * n, dst, n);
+    }
+    for (int s = 0; s < nshift; ++s) { /* :289-298 */
+      if (convs[s]) continue;
+      css = c * z[iz][s] * z[iz][s];
+      convs[s] = css < rsd_sq[s] ? 1 : 0;
+      if (convs[s]) {
+        fin[s] = css;
+        conv_at[s] = k;
+      }
+    }
+    convp = convs[0];
+    if (convp)
+      for (int s = 0; s < nshift; ++s)
+        if (!convs[s]) convp = 0;
+  }
+  for (int s = nshift - 1; s >= 0; s--) /* :330-345 */
+    rsdcg[s] = sqrt(convs[s] ? fin[s] : c * z[iz][s] * z[iz][s]);
+  if (iters)
+    for (int s = 0; s < nshift; ++s) iters[s] = conv_at[s];
+  (void)nvec;
+#undef SHIFTED_A
+#undef MDAGM
+  free(r); free(ap); free(pv); free(convs); free(conv_at); free(bs);
+  free(z[0]); free(z[1]); free(rsd_sq); free(fin); free(w.t1);
+  return k;
+}

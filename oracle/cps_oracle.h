@@ -135,6 +135,15 @@ double orc_norm_sq(const double *a, size_t n);   /* Vector::NormSqNode, serial *
 int orc_matinv(double *out_full, double *in_full, const double *u, double mass,
                double stop_rsd, int max_iter, double *true_res, const orc_params *p);
 
+/* ---- (f3): DiracOp::MInvCG (d_op_base/noarch/minvcg.C:47-355) ----------- */
+/* Multi-shift CG: (A + shift[s]) psi_s = chi.  type 0 SINGLE (psi[0] =
+ * sum alpha_s psi_s), 1 MULTI (psi = nshift vectors).  rsdcg in: relative
+ * stop residuals; out: |r_s|.  iters[s]: iteration of convergence.  Returns
+ * CPS's loop count k.                                                        */
+int orc_minvcg(double *psi, const double *chi, double chi_norm, const double *shift,
+               int nshift, double *rsdcg, int type, const double *alpha, int max_num_iter,
+               int *iters, const double *u, double mass, const orc_params *p);
+
 #ifdef __cplusplus
 }
 #endif

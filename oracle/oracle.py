@@ -53,6 +53,10 @@ def lib():
         L.orc_matinv.restype = C.c_int
         L.orc_matinv.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_int,
                                  C.POINTER(C.c_double), P]
+        _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        L.orc_minvcg.restype = C.c_int
+        L.orc_minvcg.argtypes = [_dp, _dp, C.c_double, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
+                                 _ip, _dp, C.c_double, P]
         L.orc_norm_sq.restype = C.c_double
         L.orc_norm_sq.argtypes = [_dp, C.c_size_t]
         _lib = L
@@ -138,6 +142,20 @@ class Oracle:
         it = lib().orc_matinv(out, src, self.u, mass, stop_rsd, max_iter,
                               C.byref(tr), C.byref(self.p))
         return out, it, tr.value
+
+    def minvcg(self, chi, mass, shifts, rsd, multi=True, alpha=None, max_iter=10000):
+        """DiracOp::MInvCG: returns (psi [nshift, n] or [n], k, |r_s|, iters_s)."""
+        shifts = np.ascontiguousarray(shifts, np.float64)
+        ns = shifts.size
+        rs = np.ascontiguousarray(np.broadcast_to(rsd, (ns,)), np.float64).copy()
+        al = np.ones(ns) if alpha is None else np.ascontiguousarray(alpha, np.float64)
+        psi = np.zeros((ns if multi else 1) * self.n)
+        its = np.zeros(ns, dtype=np.int32)
+        chi = np.ascontiguousarray(chi, np.float64)
+        chi_norm = float(lib().orc_norm_sq(chi, chi.size))
+        k = lib().orc_minvcg(psi, chi, chi_norm, shifts, ns, rs, 1 if multi else 0, al,
+                             max_iter, its, self.u, mass, C.byref(self.p))
+        return psi.reshape(ns, self.n) if multi else psi, k, rs, its
 
     def kappas(self):
         p = self.p

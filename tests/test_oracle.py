@@ -278,3 +278,31 @@ def test_gauge_is_su3():
     assert np.abs(np.linalg.det(U) - 1).max() < 1e-14
     # deterministic
     assert np.array_equal(U, lat.random_gauge((4, 4, 2, 2), 0.4, 3).reshape(-1, 3, 3))
+
+
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
+def test_minvcg_solves_every_shift(action):
+    """MInvCG: each psi_s solves (A + shift_s) psi_s = chi to its stop
+    residual (checked with the pinned MatPcDagMatPc), SINGLE is the alpha sum
+    of MULTI, and with one zero shift it takes InvCg's number of updates."""
+    dims, ls, mass = (4, 4, 2, 4), 4, 0.1
+    U, o = setup(dims, ls, action)
+    chi = lat.to_reals(lat.gaussian_spinor((ls, int(np.prod(dims)) // 2), 21))
+    shifts = np.array([0.0, 0.01, 0.1, 0.5])
+    psi, k, rs, its = o.minvcg(chi, mass, shifts, 1e-10, multi=True)
+    nchi = np.linalg.norm(chi)
+    for s, sh in enumerate(shifts):
+        ax, _ = o.matpcdagmatpc(psi[s], mass)
+        res = np.linalg.norm(chi - ax - sh * psi[s]) / nchi
+        assert res < 1e-9, (s, res)
+        assert 0 < its[s] < k and rs[s] < 1e-10 * nchi
+    assert its[0] >= its[1] >= its[2] >= its[3]  # larger shift, faster
+    alpha = np.array([0.3, -1.2, 0.7, 2.0])
+    single, k1, _, _ = o.minvcg(chi, mass, shifts, 1e-10, multi=False, alpha=alpha)
+    assert k1 == k
+    assert rel(single, (alpha[:, None] * psi).sum(0)) < 1e-12
+    # one zero shift: the same CG updates as InvCg (which counts one more)
+    p1, k0, _, _ = o.minvcg(chi, mass, [0.0], 1e-10)
+    sol, it, _, _ = o.invcg(chi, mass, stop_rsd=1e-10)
+    assert k0 == it - 1
+    assert rel(p1[0], sol) < 1e-12
