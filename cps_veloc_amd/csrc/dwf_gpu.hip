@@ -1351,6 +1351,7 @@ int invfcg_ptc(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec& 
 
 }  // namespace
 
+
 // ===========================================================================
 // Many right-hand sides (DWF): batched operator and the batched
 // reliable-update CG with device-side retirement and a graph-captured step
@@ -1775,6 +1776,152 @@ int invcg_mrhs(dwfgpu_s* h, const dwfgpu_cg_arg* arg, int nrhs, const double* ma
   }
   for (int j = 0; j < nrhs; ++j) iters[j] = its[j] + 1;
   HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Multi-shift CG (DiracOp::MInvCG, d_op_base/noarch/minvcg.C:47-355), FP64
+// ===========================================================================
+namespace {
+
+// psi: MULTI nshift vectors (mvec layout), SINGLE one vector; scalars and
+// stop tests exactly as minvcg.C, the vector updates of all shifts fused
+// into one pass per iteration (k_minv_update).
+int minvcg_run(dwfgpu_s* h, double mass, const double* shift, int nshift, const DVec& psi,
+               int multi, const double* alpha, const DVec& chi, double chi_norm, double* rsdcg,
+               int max_num_iter, int* iters, int* k_out) {
+  typedef Ops<P64> H;
+  TRY(ensure_work<P64>(h, 2));
+  const DVec& r = h->f[DWFGPU_F64].work[0];
+  const DVec& ap = h->f[DWFGPU_F64].work[1];
+  DVec pv;
+  TRY(alloc_mvec<P64>(h, nshift, &pv));
+  struct Free {
+    DVec* v;
+    ~Free() { free_vec(v); }
+  } fr{&pv};
+  const size_t vbytes = h->n / 2 * sizeof(cx<double>);
+  dwfk::ShiftArgs<P64> A;
+  std::memset(&A, 0, sizeof A);
+  A.nshift = nshift;
+  A.single = multi ? 0 : 1;
+  for (int s = 0; s < nshift; ++s) {
+    A.p[s] = (cx<double>*)mslice<P64>(h, pv, s).d;
+    A.psi[s] = (cx<double>*)(multi ? mslice<P64>(h, psi, s).d : psi.d);
+  }
+  auto launch = [&](unsigned psi_mask, unsigned p_mask) -> int {
+    A.psi_mask = psi_mask;
+    A.p_mask = p_mask;
+    hipLaunchKernelGGL((dwfk::k_minv_update<P64>), blas_grid(), blas_block(), 0, h->st, A,
+                       cfld<P64>(r), h->g);
+    HIPCHK(hipGetLastError());
+    return DWFGPU_OK;
+  };
+  // d = <p0, (A + shift0) p0>
+  DVec p0 = mslice<P64>(h, pv, 0);
+  auto shifted_a = [&](double* d) -> int {
+    if (shift[0] > 0) {
+      TRY(H::mdagm(h, mass, ap, p0, nullptr));
+      TRY(H::axpby(h, ap, p0, shift[0], 1.0));
+      return dot<P64>(h, p0, ap, d);
+    }
+    return H::mdagm(h, mass, ap, p0, d);
+  };
+  if (multi) HIPCHK(hipMemsetAsync(psi.d, 0, (size_t)nshift * vbytes, h->st));  // :118-121
+  HIPCHK(hipMemcpyAsync(r.d, chi.d, vbytes, hipMemcpyDeviceToDevice, h->st));
+  for (int s = 0; s < nshift; ++s)
+    HIPCHK(hipMemcpyAsync(A.p[s], chi.d, vbytes, hipMemcpyDeviceToDevice, h->st));
+  double cp = chi_norm;
+  std::vector<double> rsd_sq(nshift), bs(nshift), z0(nshift), z1(nshift), fin(nshift);
+  double* z[2] = {z0.data(), z1.data()};
+  std::vector<int> convs(nshift, 0), conv_at(nshift, 0);
+  for (int s = 0; s < nshift; ++s) rsd_sq[s] = cp * rsdcg[s] * rsdcg[s];
+  double d;
+  TRY(shifted_a(&d));
+  double b = -cp / d;
+  z[0][0] = 1.0;
+  z[1][0] = 1.0;
+  bs[0] = b;
+  int iz = 1;
+  for (int s = 1; s < nshift; ++s) {
+    z[1 - iz][s] = 1.0;
+    z[iz][s] = 1.0 / (1.0 - b * (shift[s] - shift[0]));
+    bs[s] = b * z[iz][s];
+  }
+  TRY(H::axpy_to(h, r, ap, r, b, true));  // r += b A p0
+  double c;
+  TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &c));
+  // the solution update of the first step is pending (psi_s -= bs_s chi with
+  // p_s = chi): it is applied with the next direction update or at the end
+  unsigned pending = 0;
+  for (int s = 0; s < nshift; ++s) {
+    A.fpsi[s] = multi ? -bs[s] : -bs[s] * alpha[s];
+    pending |= 1u << s;
+  }
+  int convp = c < rsd_sq[0] ? 1 : 0;
+  double a = c / cp;
+  int k;
+  for (k = 1; k <= max_num_iter && !convp; k++) {
+    a = c / cp;
+    unsigned pmask = 0;
+    for (int s = 0; s < nshift; ++s) {
+      if (convs[s]) continue;
+      pmask |= 1u << s;
+      if (s == 0) {
+        A.zr[0] = 1.0;
+        A.ap[0] = a;
+      } else {
+        A.zr[s] = z[iz][s];
+        A.ap[s] = a * (z[iz][s] * bs[s]) / (z[1 - iz][s] * b);
+      }
+    }
+    TRY(launch(pending, pmask));
+    pending = 0;
+    cp = c;
+    TRY(shifted_a(&d));
+    const double bp = b;
+    b = -cp / d;
+    bs[0] = b;
+    iz = 1 - iz;
+    for (int s = 1; s < nshift; ++s) {
+      if (convs[s]) continue;
+      const double ztmp = z[1 - iz][s] * z[iz][s] * bp /
+                          (b * a * (z[iz][s] - z[1 - iz][s]) +
+                           z[iz][s] * bp * (1 - b * (shift[s] - shift[0])));
+      bs[s] = b * ztmp / z[1 - iz][s];
+      z[iz][s] = ztmp;
+    }
+    TRY(H::axpy_to(h, r, ap, r, b, true));
+    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &c));
+    for (int s = 0; s < nshift; ++s) {
+      if (convs[s]) continue;
+      A.fpsi[s] = multi ? -bs[s] : -bs[s] * alpha[s];
+      pending |= 1u << s;
+    }
+    for (int s = 0; s < nshift; ++s) {
+      if (convs[s]) continue;
+      const double css = c * z[iz][s] * z[iz][s];
+      convs[s] = css < rsd_sq[s] ? 1 : 0;
+      if (convs[s]) {
+        fin[s] = css;
+        conv_at[s] = k;
+      }
+    }
+    convp = convs[0];
+    if (convp)
+      for (int s = 0; s < nshift; ++s)
+        if (!convs[s]) convp = 0;
+  }
+  if (pending) TRY(launch(pending, 0u));
+  HIPCHK(hipStreamSynchronize(h->st));
+  for (int s = nshift - 1; s >= 0; s--) rsdcg[s] = std::sqrt(convs[s] ? fin[s] : c * z[iz][s] * z[iz][s]);
+  if (iters)
+    for (int s = 0; s < nshift; ++s) iters[s] = conv_at[s];
+  if (k_out) *k_out = k;
+  for (int s = 0; s < nshift; ++s)
+    if (!convs[s]) return DWFGPU_MAXITER;
   return DWFGPU_OK;
 }
 
@@ -2459,6 +2606,48 @@ int dwfgpu_invcg_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, doubl
       set_err("dwfgpu_invcg_ptc_dev: sloppy precision must be F32, BF16 or I16");
       return DWFGPU_BAD_ARG;
   }
+}
+
+int dwfgpu_minvcg_dev(dwfgpu_t h, double mass, const double* shifts, int nshift,
+                      dwfgpu_mvec_t psi, dwfgpu_vec_t chi, double chi_norm, double* rsdcg,
+                      int type, const double* alpha, int max_num_iter, int* iters, int* k) {
+  if (!h || !shifts || !psi || !chi || !rsdcg || psi->h != h || chi->h != h) return DWFGPU_BAD_ARG;
+  if (h->prec != DWFGPU_F64 || nshift < 1 || nshift > dwfk::MAXSHIFT ||
+      (type != DWFGPU_SHIFT_SINGLE && type != DWFGPU_SHIFT_MULTI) ||
+      (type == DWFGPU_SHIFT_SINGLE && !alpha) ||
+      psi->nrhs != (type == DWFGPU_SHIFT_MULTI ? nshift : 1) || max_num_iter < 1) {
+    set_err("dwfgpu_minvcg: FP64 handle, 1..%d shifts, type SINGLE (with alpha, one "
+            "vector) or MULTI (nshift vectors) and max_num_iter >= 1 needed", dwfk::MAXSHIFT);
+    return DWFGPU_BAD_ARG;
+  }
+  if (chi_norm <= 0.0) TRY(Ops<P64>::norm(h, chi->v, &chi_norm));
+  return minvcg_run(h, mass, shifts, nshift, psi->v, type == DWFGPU_SHIFT_MULTI, alpha, chi->v,
+                    chi_norm, rsdcg, max_num_iter, iters, k);
+}
+
+int dwfgpu_minvcg(dwfgpu_t h, double mass, const double* shifts, int nshift, double* psi,
+                  const double* chi, double chi_norm, double* rsdcg, int type,
+                  const double* alpha, int max_num_iter, int* iters, int* k) {
+  if (!h || !psi || !chi) return DWFGPU_BAD_ARG;
+  const int nv = type == DWFGPU_SHIFT_MULTI ? nshift : 1;
+  if (nv < 1) return DWFGPU_BAD_ARG;
+  dwfgpu_mvec_t mv = nullptr;
+  dwfgpu_vec_t cv = nullptr;
+  TRY(dwfgpu_mvec_create(h, nv, &mv));
+  int st = dwfgpu_vec_create(h, &cv);
+  if (!st) st = dwfgpu_vec_upload(cv, chi);
+  if (!st && type == DWFGPU_SHIFT_SINGLE) st = dwfgpu_mvec_upload(mv, 0, psi);
+  if (!st)
+    st = dwfgpu_minvcg_dev(h, mass, shifts, nshift, mv, cv, chi_norm, rsdcg, type, alpha,
+                           max_num_iter, iters, k);
+  if (st == DWFGPU_OK || st == DWFGPU_MAXITER)
+    for (int j = 0; j < nv; ++j) {
+      const int r = dwfgpu_mvec_download(mv, j, psi + (size_t)j * h->n);
+      if (r) st = r;
+    }
+  if (cv) dwfgpu_vec_destroy(cv);
+  if (mv) dwfgpu_mvec_destroy(mv);
+  return st;
 }
 
 }  // extern "C"

@@ -1194,6 +1194,62 @@ k_compress_links(typename P::S* __restrict__ dst, const cx<double>* __restrict__
 
 
 // ---------------------------------------------------------------------------
+// Multi-shift CG (DiracOp::MInvCG, d_op_base/noarch/minvcg.C:47-355): the
+// updates of every shift in one pass per iteration.  For each shift s with
+// bit s of psi_mask: psi_s += fpsi[s] p_s (the solution update that ends an
+// iteration, :266-285); then with bit s of p_mask: p_s = zr[s] r + ap[s] p_s
+// (the next iteration's direction, :210-222).  SINGLE keeps one psi that
+// receives every shift's term.
+// ---------------------------------------------------------------------------
+constexpr int MAXSHIFT = 32;
+
+template <class P>
+struct ShiftArgs {
+  typename P::C* psi[MAXSHIFT];
+  typename P::C* p[MAXSHIFT];
+  double fpsi[MAXSHIFT], zr[MAXSHIFT], ap[MAXSHIFT];
+  unsigned psi_mask, p_mask;
+  int nshift, single;
+};
+
+template <class P>
+__global__ void __launch_bounds__(BLAS_THREADS)
+k_minv_update(ShiftArgs<P> a, CFld<P> r, Geom g) {
+  typedef typename P::R R;
+  DWFK_SITE_LOOP(k, g) {
+    cx<R> rv[12], acc[12];
+    ld_site<P>(rv, r, g, k);
+    if (a.single) ld_site<P>(acc, CFld<P>{a.psi[0], nullptr}, g, k);
+    for (int s = 0; s < a.nshift; ++s) {
+      const bool up_psi = (a.psi_mask >> s) & 1u, up_p = (a.p_mask >> s) & 1u;
+      if (!up_psi && !up_p) continue;
+      cx<R> pv[12];
+      ld_site<P>(pv, CFld<P>{a.p[s], nullptr}, g, k);
+      if (up_psi) {
+        const R f = (R)a.fpsi[s];
+        if (a.single) {
+#pragma unroll
+          for (int c = 0; c < 12; ++c) acc[c] = f * pv[c] + acc[c];
+        } else {
+          cx<R> ps[12];
+          ld_site<P>(ps, CFld<P>{a.psi[s], nullptr}, g, k);
+#pragma unroll
+          for (int c = 0; c < 12; ++c) ps[c] = f * pv[c] + ps[c];
+          st_site_k<P>(Fld<P>{a.psi[s], nullptr}, g, k, ps);
+        }
+      }
+      if (up_p) {
+        const R zr = (R)a.zr[s], ap = (R)a.ap[s];
+#pragma unroll
+        for (int c = 0; c < 12; ++c) pv[c] = zr * rv[c] + ap * pv[c];
+        st_site_k<P>(Fld<P>{a.p[s], nullptr}, g, k, pv);
+      }
+    }
+    if (a.single) st_site_k<P>(Fld<P>{a.psi[0], nullptr}, g, k, acc);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Many right-hand sides (DWF): one launch applies the operator to every
 // active RHS.  A work-group holds ns sites x Ls slices x nr RHS (threadIdx.z);
 // the links staged in LDS serve all of them.  Vectors of RHS j start at

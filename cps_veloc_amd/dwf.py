@@ -43,7 +43,7 @@ EXPORTS = (
     "dwfgpu_matpcdagmatpc_mrhs_dev", "dwfgpu_invcg_mrhs_dev", "dwfgpu_mrhs_shape",
     "dwfgpu_ptc_num_weights", "dwfgpu_ptc_init", "dwfgpu_ptc_set_weights",
     "dwfgpu_ptc_get_weights", "dwfgpu_ptc_apply", "dwfgpu_ptc_loss_grad", "dwfgpu_ptc_train",
-    "dwfgpu_invcg_ptc_dev",
+    "dwfgpu_invcg_ptc_dev", "dwfgpu_minvcg", "dwfgpu_minvcg_dev",
     "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
 )
 # every entry point of include/dwf_synth.h
@@ -136,6 +136,10 @@ def lib():
         L.dwfgpu_ptc_loss_grad.argtypes = [H, _dp, _dp, C.POINTER(C.c_double), _fp]
         L.dwfgpu_ptc_train.argtypes = [H, C.POINTER(PtcTrainArgC), C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.dwfgpu_minvcg.argtypes = [H, C.c_double, _dp, C.c_int, _dp, _dp, C.c_double, _dp,
+                                    C.c_int, C.c_void_p, C.c_int, _ip, C.POINTER(C.c_int)]
+        L.dwfgpu_minvcg_dev.argtypes = [H, C.c_double, _dp, C.c_int, H, H, C.c_double, _dp,
+                                        C.c_int, C.c_void_p, C.c_int, _ip, C.POINTER(C.c_int)]
         L.dwfgpu_invcg_ptc_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, H, H,
                                            C.POINTER(C.c_double), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), C.POINTER(C.c_int)]
@@ -344,6 +348,26 @@ class _DiracOpGpu:
                                          1 if eager else 0, _arr(masses), sol.h, src.h, tr, it,
                                          nr, stv), "InvCgMrhs_dev")
         return it, tr, stv, nr
+
+    # --- multi-shift CG (DiracOp::MInvCG) -----------------------------------
+    def MInvCG(self, chi, mass, shifts, rsd, multi=True, alpha=None, max_iter=10000,
+               psi=None):
+        """Returns (psi [nshift, n] (MULTI) or [n] (SINGLE), k, |r_s|, iters_s, status)."""
+        shifts = np.ascontiguousarray(shifts, np.float64)
+        ns = shifts.size
+        rs = np.ascontiguousarray(np.broadcast_to(rsd, (ns,)), np.float64).copy()
+        al = None if alpha is None else np.ascontiguousarray(alpha, np.float64)
+        if psi is None:
+            psi = np.zeros((ns if multi else 1) * self.n)
+        psi = np.ascontiguousarray(psi, np.float64).reshape(-1)
+        its = np.zeros(ns, dtype=np.int32)
+        k = C.c_int()
+        st = _chk(lib().dwfgpu_minvcg(self.h, mass, shifts, ns, psi, _arr(chi), -1.0, rs,
+                                      1 if multi else 0,
+                                      None if al is None else al.ctypes.data_as(C.c_void_p),
+                                      max_iter, its, C.byref(k)), "MInvCG",
+                  allow=(OK, MAXITER))
+        return (psi.reshape(ns, self.n) if multi else psi), k.value, rs, its, st
 
     # --- learned preconditioner -------------------------------------------
     def ptc_init(self, seed=1, scale=0.02):

@@ -162,6 +162,28 @@ int dwfgpu_invcg_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, doub
 /* the launch shape (sites, RHS per work-group) the last batched call used */
 int dwfgpu_mrhs_shape(dwfgpu_t h, int *ns, int *nr);
 
+/* ---- multi-shift CG ------------------------------------------------------ */
+/* DiracOp::MInvCG(psi, chi, chi_norm, shift, Nshift, isz, RsdCG, type, alpha)
+ * (include/util/dirac_op.h:222, d_op_base/noarch/minvcg.C:47-355): solves
+ * (MatPcDagMatPc + shifts[s]) psi_s = chi for all shifts with one operator
+ * application per iteration, in FP64 with CPS's recurrences and stop tests.
+ * type DWFGPU_SHIFT_MULTI: psi holds nshift vectors (zeroed first);
+ * DWFGPU_SHIFT_SINGLE: psi is one vector that receives sum_s alpha[s] psi_s
+ * (added to its entry value, as CPS does).  GENERAL is not supported.
+ * chi_norm <= 0: |chi|^2 is computed.  rsdcg[s]: relative stop residual on
+ * entry, |r_s| on return (minvcg.C:330-345); iters[s] (may be NULL): the
+ * iteration at which shift s converged (0: it did not); *k: CPS's return
+ * value.  Returns DWFGPU_MAXITER when some shift did not converge.
+ * At most 32 shifts; FP64 handles only.                                      */
+#define DWFGPU_SHIFT_SINGLE 0 /* MultiShiftSolveType SINGLE (include/alg/enum_int.h:233) */
+#define DWFGPU_SHIFT_MULTI 1  /* MULTI */
+int dwfgpu_minvcg(dwfgpu_t h, double mass, const double *shifts, int nshift, double *psi,
+                  const double *chi, double chi_norm, double *rsdcg, int type,
+                  const double *alpha, int max_num_iter, int *iters, int *k);
+int dwfgpu_minvcg_dev(dwfgpu_t h, double mass, const double *shifts, int nshift,
+                      dwfgpu_mvec_t psi, dwfgpu_vec_t chi, double chi_norm, double *rsdcg,
+                      int type, const double *alpha, int max_num_iter, int *iters, int *k);
+
 /* ---- learned preconditioner (no CPS counterpart) ------------------------ */
 /* A gauge-equivariant parallel-transport network z = |r| N(r/|r|) acting on
  * CG vectors: three layers Ls -> 16 -> 16 -> Ls channels (Ls in {4,8,12,16}),

@@ -171,3 +171,26 @@ def test_device_vectors(action):
     ms, msk, byk = g.time_MatPcDagMatPc(vout, vin, 0.04, 3)
     assert ms > 0 and len(msk) == 5
     assert np.array_equal(vout.download(), host)
+
+
+@pytest.mark.parametrize("action,dims,ls", [("dwf", (4, 4, 4, 8), 8), ("mobius", (4, 2, 6, 4), 6)])
+def test_minvcg_matches_oracle(action, dims, ls):
+    say(f"[gpu] MInvCG {action} {dims} ls={ls}")
+    U, o, g = make(action, dims, ls)
+    chi = src_vec(dims, ls, 41)
+    mass = 0.05
+    shifts = np.array([0.0, 0.003, 0.02, 0.1, 0.6])
+    gp, gk, grs, git, gst = g.MInvCG(chi, mass, shifts, 1e-10)
+    op, ok, ors, oit = o.minvcg(chi, mass, shifts, 1e-10)
+    say(f"  k gpu={gk} oracle={ok}; per-shift iters gpu={git.tolist()} oracle={oit.tolist()}")
+    assert gst == 0 and gk == ok and np.array_equal(git, oit)
+    assert np.allclose(grs, ors, rtol=1e-3)
+    for s in range(len(shifts)):
+        assert np.linalg.norm(gp[s] - op[s]) / np.linalg.norm(op[s]) < 1e-9
+    # SINGLE: the alpha sum, added to the vector passed in
+    alpha = np.array([0.5, -1.0, 2.0, 0.25, 1.5])
+    start = src_vec(dims, ls, 42)
+    sp, sk, _, _, sst = g.MInvCG(chi, mass, shifts, 1e-10, multi=False, alpha=alpha, psi=start)
+    assert sst == 0 and sk == gk
+    want = start + (alpha[:, None] * gp).sum(0)
+    assert np.linalg.norm(sp - want) / np.linalg.norm(want) < 1e-10
