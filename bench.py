@@ -176,6 +176,9 @@ def main():
     ap.add_argument("--delta", type=float, default=0.1,
                     help="reliable update when |r| drops by this factor")
     ap.add_argument("--mrhs-sloppy", default="f32", choices=["f32", "bf16", "i16"])
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip training the network preconditioner and its solve")
+    ap.add_argument("--learned-sloppy", default="i16", choices=["f32", "bf16", "i16"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU "
@@ -257,8 +260,35 @@ def main():
                 f"({t_plain / ts:.2f}x)")
             if st == 0 and tr <= 1e-10 and (best is None or ts < best[1]):
                 best = (p, ts)
+        if not args.no_learned:
+            # the learned preconditioner: train on the GPU, then flexible CG
+            log("[bench] training the network preconditioner")
+            op.ptc_init(5, 0.02)
+            t1 = time.perf_counter()
+            l0, l1, npairs = op.ptc_train(mass, nsolves=6, pairs_per_solve=0, random_pairs=32,
+                                          stride=3, epochs=3, batch=4, lr=2e-3, seed=17)
+            t_train = time.perf_counter() - t1
+            vsol.upload(np.zeros(op.n))
+            t1 = time.perf_counter()
+            it, tr, st, nrel, nnet = op.InvCgPtc_dev(vsol, vin, arg, args.learned_sloppy,
+                                                     args.delta)
+            ts = time.perf_counter() - t1
+            err = float(np.linalg.norm(vsol.download() - x_plain)) / nx
+            solve["learned_fcg"] = {"seconds": round(ts, 4), "iters": it, "net_steps": nnet,
+                                    "true_res": tr, "status": st, "reliable_updates": nrel,
+                                    "sloppy": args.learned_sloppy,
+                                    "rel_err_vs_plain_fp64": err,
+                                    "speedup_vs_plain_fp64": round(t_plain / ts, 3),
+                                    "training": {"seconds": round(t_train, 3), "pairs": npairs,
+                                                 "loss_first": l0, "loss_last": l1}}
+            log(f"[bench] network: trained on {npairs} pairs in {t_train:.1f}s (loss "
+                f"{l0:.3f} -> {l1:.3f}); flexible CG {it} iters ({nnet} with the net), "
+                f"true res {tr:.3e}, {ts:.2f}s ({t_plain / ts:.2f}x)")
+            if st == 0 and tr <= 1e-10 and (best is None or ts < best[1]):
+                best = ("learned", ts)
         if best:
-            solve["best"] = {"method": f"mixed_{best[0]}_cg",
+            name = "learned_fcg" if best[0] == "learned" else f"mixed_{best[0]}_cg"
+            solve["best"] = {"method": name,
                              "speedup_vs_plain_fp64": round(t_plain / best[1], 3)}
         vsol.free()
 
