@@ -1978,8 +1978,17 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
   }
   // t and z tiles of 4 slices when possible; 8 XCDs on gfx950 (a different
   // count only changes the work-group order, not the result)
-  const int bt = dims[3] % 4 == 0 ? 4 : 2;
-  const int bz = dims[2] % 4 == 0 ? 4 : 2;
+  int bt = dims[3] % 4 == 0 ? 4 : 2;
+  int bz = dims[2] % 4 == 0 ? 4 : 2;
+  // DWFGPU_TILE="bt,bz" overrides the tile (each must divide its extent)
+  if (const char* env = getenv("DWFGPU_TILE")) {
+    int et = 0, ez = 0;
+    if (sscanf(env, "%d,%d", &et, &ez) == 2 && et > 0 && ez > 0 && dims[3] % et == 0 &&
+        dims[2] % ez == 0) {
+      bt = et;
+      bz = ez;
+    }
+  }
   const int bcmask = h->bc[0] | h->bc[1] << 1 | h->bc[2] << 2 | h->bc[3] << 3;
   h->g = Geom{dims[0], dims[1], dims[2], dims[3], dims[0] / 2, (int)(v4 / 2), ls, bt, bz, 8,
               bcmask};
