@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "dwf_gpu.h"
@@ -86,6 +87,7 @@ struct dwfgpu_s {
   hipStream_t st = nullptr;
   PrecFields f[NPREC];
   void* cps_stage = nullptr;  // FP64 buffer in CPS order (host-array API)
+  double* stage4d = nullptr;  // one 4D CANONICAL fermion slice (format calls)
   double* partial = nullptr;
   int npartial = 0;
   double* dscal = nullptr;
@@ -628,6 +630,25 @@ struct Ops {
     HIPCHK(hipStreamSynchronize(h->st));
     return DWFGPU_OK;
   }
+
+  // one 4D CANONICAL slice on the device <-> slice s of the halves of a full
+  // vector (up / lo: which spin pairs are taken; canon_to_halves zeroes the rest)
+  static int canon_to_halves(dwfgpu_s* h, const DVec& first, const DVec& second,
+                             const double* canon, int s, int up, int lo) {
+    hipLaunchKernelGGL((dwfk::k_canon_to_halves<P>), dim3(1024), dim3(256), 0, h->st,
+                       fld<P>(first), fld<P>(second), canon, h->g, s,
+                       h->action == DWFGPU_DWF ? 1 : 0, up, lo);
+    HIPCHK(hipGetLastError());
+    return DWFGPU_OK;
+  }
+  static int halves_to_canon(dwfgpu_s* h, double* canon, const DVec& first, const DVec& second,
+                             int s, int up, int lo) {
+    hipLaunchKernelGGL((dwfk::k_halves_to_canon<P>), dim3(1024), dim3(256), 0, h->st, canon,
+                       cfld<P>(first), cfld<P>(second), h->g, s,
+                       h->action == DWFGPU_DWF ? 1 : 0, up, lo);
+    HIPCHK(hipGetLastError());
+    return DWFGPU_OK;
+  }
 };
 
 // FP64-reliable-update CG with the bulk of MatPcDagMatPc in sloppy policy PS.
@@ -637,11 +658,16 @@ struct Ops {
 // |r_s| falls below delta times |r| at the previous update, and whenever the
 // sloppy residual meets the stop condition; the solve ends only when the
 // FP64 true residual meets it (stop rule of inv_cg.C:174,354).
+// The correction x_s is kept in FP32, or in int16 with per-site scales when
+// the sloppy vectors are int16: it only has to be accurate relative to the
+// residual at the last reliable update, and its rounding (1.5e-5 of the site
+// maximum per step) stays far below the factor delta each cycle gains.
 template <class PS>
 int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec& sol,
                 const DVec& src, double* true_res, int* iters, int* nreliable) {
   typedef Ops<P64> H;
   typedef Ops<PS> L;
+  typedef typename std::conditional<std::is_same<PS, PI16>::value, PI16, P32>::type XS;
   TRY(ensure_work<P64>(h, 2));
   TRY(ensure_work<PS>(h, 3));
   TRY(ensure_work<P32>(h, 4));
@@ -649,7 +675,8 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   const DVec& ax = h->f[DWFGPU_F64].work[1];
   const DVec& rs = h->f[pidx<PS>()].work[0];
   const DVec& ps = h->f[pidx<PS>()].work[1];
-  const DVec& xs = h->f[DWFGPU_F32].work[3];
+  const DVec& xs = std::is_same<XS, PI16>::value ? h->f[pidx<PS>()].work[2]
+                                                  : h->f[DWFGPU_F32].work[3];
   const double mass = arg->mass;
   double src_norm_sq;
   TRY(H::norm(h, src, &src_norm_sq));
@@ -663,7 +690,8 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
                      cfld<P64>(r), h->g);
   HIPCHK(hipGetLastError());
   TRY(L::copy(h, ps, rs));
-  HIPCHK(hipMemsetAsync(xs.d, 0, h->n / 2 * sizeof(cx<float>), h->st));
+  HIPCHK(hipMemsetAsync(xs.d, 0, h->n / 2 * sizeof(typename XS::S), h->st));
+  if (xs.sc) HIPCHK(hipMemsetAsync(xs.sc, 0, (size_t)h->g.vh * h->ls * sizeof(float), h->st));
   double rr_update = rr;  // |r|^2 at the last reliable update
   int itr = 0, nrel = 0, status = DWFGPU_OK;
   const int max_itr = rr <= stp_cnd ? 0 : arg->max_num_iter - 1;
@@ -671,16 +699,22 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   while (!converged && itr < max_itr) {
     itr = itr + 1;
     double d, rr_new;
-    // dscal[0] holds rr; the fused step updates xs, rs and dscal[0]
-    TRY(L::template mdagm_cg<P32>(h, mass, ps, rs, xs, &rr_new, &d));
+    // dscal[0] holds rr; the fused step updates rs and dscal[0]; x_s += a p_s
+    // rides in the pass that forms the next direction
+    TRY(L::template mdagm_cg<dwfk::NoX>(h, mass, ps, rs, DVec(), &rr_new, &d));
     if (d == 0.0) {
       status = DWFGPU_BREAKDOWN;
       break;
     }
+    const double a = rr / d;  // the alpha the hop epilogue used
+    bool reliable = false;
     if (rr_new <= stp_cnd || rr_new < delta * delta * rr_update) {
       // reliable update in FP64
-      hipLaunchKernelGGL((dwfk::k_accumulate<P32>), blas_grid(), blas_block(), 0, h->st,
-                         fld<P64>(sol), fld<P32>(xs), h->g);
+      reliable = true;
+      hipLaunchKernelGGL((dwfk::k_cg_xp<PS, XS, false>), blas_grid(), blas_block(), 0, h->st,
+                         fld<XS>(xs), fld<PS>(ps), cfld<PS>(rs), a, 0.0, h->g);
+      hipLaunchKernelGGL((dwfk::k_accumulate<XS>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P64>(sol), fld<XS>(xs), h->g);
       HIPCHK(hipGetLastError());
       TRY(H::mdagm(h, mass, ax, sol, nullptr));
       TRY(H::axpy_to(h, r, ax, src, -1.0, true));
@@ -697,13 +731,20 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
       }
     }
     const double b = rr_new / rr;
-    TRY(L::axpby(h, ps, rs, 1.0, b));  // p_s = r_s + b p_s
+    if (reliable) {
+      TRY(L::axpby(h, ps, rs, 1.0, b));  // p_s = r_s + b p_s
+    } else {
+      // x_s += a p_s ; p_s = r_s + b p_s
+      hipLaunchKernelGGL((dwfk::k_cg_xp<PS, XS, true>), blas_grid(), blas_block(), 0, h->st,
+                         fld<XS>(xs), fld<PS>(ps), cfld<PS>(rs), a, b, h->g);
+      HIPCHK(hipGetLastError());
+    }
     rr = rr_new;
   }
   if (!converged && status == DWFGPU_OK) {
     // fold the pending correction in and measure the true residual
-    hipLaunchKernelGGL((dwfk::k_accumulate<P32>), blas_grid(), blas_block(), 0, h->st,
-                       fld<P64>(sol), fld<P32>(xs), h->g);
+    hipLaunchKernelGGL((dwfk::k_accumulate<XS>), blas_grid(), blas_block(), 0, h->st,
+                       fld<P64>(sol), fld<XS>(xs), h->g);
     HIPCHK(hipGetLastError());
     TRY(H::mdagm(h, mass, ax, sol, nullptr));
     TRY(H::axpy_to(h, r, ax, src, -1.0, true));
@@ -828,7 +869,7 @@ void free_all(dwfgpu_s* h) {
     for (auto& w : F.work) free_vec(&w);
     F.ready = false;
   }
-  void* ptrs[] = {h->cps_stage, h->partial, h->dscal};
+  void* ptrs[] = {h->cps_stage, h->stage4d, h->partial, h->dscal};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->hscal) (void)hipHostFree(h->hscal);
@@ -2657,6 +2698,109 @@ int dwfgpu_minvcg(dwfgpu_t h, double mass, const double* shifts, int nshift, dou
   if (cv) dwfgpu_vec_destroy(cv);
   if (mv) dwfgpu_mvec_destroy(mv);
   return st;
+}
+
+// ---- host formats of whole fermion fields (convert.C:639-831,
+// f_dwf_base.C:961-1168) ----------------------------------------------------
+namespace {
+int ensure_stage4d(dwfgpu_s* h) {
+  if (!h->stage4d) TRY(alloc(h, (void**)&h->stage4d, (size_t)2 * h->g.vh * 24 * sizeof(double)));
+  return DWFGPU_OK;
+}
+bool own_pair(dwfgpu_t h, dwfgpu_vec_t a, dwfgpu_vec_t b) {
+  return h && a && b && a->h == h && b->h == h && a != b;
+}
+}  // namespace
+
+int dwfgpu_fermion_from_canonical(dwfgpu_t h, dwfgpu_vec_t first, dwfgpu_vec_t second,
+                                  const double* canon) {
+  if (!own_pair(h, first, second) || !canon) return DWFGPU_BAD_ARG;
+  TRY(ensure_stage4d(h));
+  const size_t v4r = (size_t)2 * h->g.vh * 24;
+  for (int s = 0; s < h->ls; ++s) {
+    HIPCHK(hipMemcpyAsync(h->stage4d, canon + (size_t)s * v4r, v4r * sizeof(double),
+                          hipMemcpyHostToDevice, h->st));
+    TRY(DISPATCH(h, canon_to_halves(h, first->v, second->v, h->stage4d, s, 1, 1)));
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
+}
+
+int dwfgpu_fermion_to_canonical(dwfgpu_t h, double* canon, dwfgpu_vec_t first,
+                                dwfgpu_vec_t second) {
+  if (!own_pair(h, first, second) || !canon) return DWFGPU_BAD_ARG;
+  TRY(ensure_stage4d(h));
+  const size_t v4r = (size_t)2 * h->g.vh * 24;
+  for (int s = 0; s < h->ls; ++s) {
+    TRY(DISPATCH(h, halves_to_canon(h, h->stage4d, first->v, second->v, s, 1, 1)));
+    HIPCHK(hipMemcpyAsync(canon + (size_t)s * v4r, h->stage4d, v4r * sizeof(double),
+                          hipMemcpyDeviceToHost, h->st));
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
+}
+
+int dwfgpu_four2five(dwfgpu_t h, dwfgpu_vec_t first, dwfgpu_vec_t second, const double* four,
+                     int s_u, int s_l) {
+  if (!own_pair(h, first, second) || !four) return DWFGPU_BAD_ARG;
+  if (s_u < 0 || s_u >= h->ls || s_l < 0 || s_l >= h->ls) return DWFGPU_BAD_ARG;
+  TRY(ensure_stage4d(h));
+  const size_t v4r = (size_t)2 * h->g.vh * 24;
+  HIPCHK(hipMemcpyAsync(h->stage4d, four, v4r * sizeof(double), hipMemcpyHostToDevice, h->st));
+  for (int s = 0; s < h->ls; ++s)
+    TRY(DISPATCH(h, canon_to_halves(h, first->v, second->v, h->stage4d, s, s == s_u ? 1 : 0,
+                                    s == s_l ? 1 : 0)));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
+}
+
+int dwfgpu_five2four(dwfgpu_t h, double* four, dwfgpu_vec_t first, dwfgpu_vec_t second,
+                     int s_u, int s_l) {
+  if (!own_pair(h, first, second) || !four) return DWFGPU_BAD_ARG;
+  if (s_u < 0 || s_u >= h->ls || s_l < 0 || s_l >= h->ls) return DWFGPU_BAD_ARG;
+  TRY(ensure_stage4d(h));
+  const size_t v4r = (size_t)2 * h->g.vh * 24;
+  HIPCHK(hipMemsetAsync(h->stage4d, 0, v4r * sizeof(double), h->st));
+  TRY(DISPATCH(h, halves_to_canon(h, h->stage4d, first->v, second->v, s_u, 1, 0)));
+  TRY(DISPATCH(h, halves_to_canon(h, h->stage4d, first->v, second->v, s_l, 0, 1)));
+  HIPCHK(hipMemcpyAsync(four, h->stage4d, v4r * sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
+}
+
+int dwfgpu_matinv_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, dwfgpu_vec_t out_first,
+                      dwfgpu_vec_t out_second, dwfgpu_vec_t in_first, dwfgpu_vec_t in_second,
+                      double* true_res, int* iters) {
+  if (!arg || !own_pair(h, out_first, out_second) || !own_pair(h, in_first, in_second))
+    return DWFGPU_BAD_ARG;
+  if (out_first == in_first || out_first == in_second || out_second == in_first ||
+      out_second == in_second)
+    return DWFGPU_BAD_ARG;
+  if (arg->max_num_iter < 1) return DWFGPU_BAD_ARG;
+  TRY(ensure_api(h, 6));
+  const int st = DISPATCH(h, matinv(h, arg, out_first->v, out_second->v, in_first->v,
+                                    in_second->v, true_res, iters));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return st;
+}
+
+int dwfgpu_plaquette(dwfgpu_t h, double* plaq, double* link_trace) {
+  if (!h || (!plaq && !link_trace)) return DWFGPU_BAD_ARG;
+  const int nb = h->npartial / 2 < 512 ? h->npartial / 2 : 512;
+  hipLaunchKernelGGL(dwfk::k_plaquette, dim3(nb), dim3(256), 0, h->st,
+                     (const cx<double>*)h->f[DWFGPU_F64].U, h->g, h->partial);
+  HIPCHK(hipGetLastError());
+  TRY(reduce_to(h, nb, 0));
+  hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(1024), 0, h->st, h->partial + nb, nb,
+                     h->dscal + 1);
+  HIPCHK(hipGetLastError());
+  double sc[2];
+  TRY(fetch_scalars(h, 2, sc));
+  const double v4 = 2.0 * h->g.vh;
+  // Lattice::SumReTrPlaq / (18 V), sum ReTr U / (12 V) (ReadLatticePar.C:403,428)
+  if (plaq) *plaq = sc[0] / (18.0 * v4);
+  if (link_trace) *link_trace = sc[1] / (12.0 * v4);
+  return DWFGPU_OK;
 }
 
 }  // extern "C"

@@ -28,6 +28,7 @@
 // Work-groups are renumbered so each XCD sweeps one contiguous site range.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <stdint.h>
 
 namespace dwfk {
@@ -253,6 +254,15 @@ __device__ __forceinline__ void ld_spinor(const typename P::C* __restrict__ d,
   for (int k = 0; k < nch<P>(); ++k) chunk_unpack<P>(d[(size_t)k * vh + n], s, v + k * P::kCPC);
 }
 
+// the same without the per-site scale (16-bit policies: raw values / 32767
+// or raw bf16; the caller applies the scale)
+template <class P>
+__device__ __forceinline__ void ld_spinor_raw(const typename P::C* __restrict__ d, int vh, int n,
+                                              cx<typename P::R>* v) {
+#pragma unroll
+  for (int k = 0; k < nch<P>(); ++k) chunk_unpack<P>(d[(size_t)k * vh + n], 1.0f, v + k * P::kCPC);
+}
+
 // store the 12 complex components of site n of a slice
 template <class P>
 __device__ __forceinline__ void st_site(typename P::C* __restrict__ d, float* __restrict__ sc,
@@ -380,31 +390,34 @@ __device__ __forceinline__ void project(const cx<R>* p, cx<R>* h) {
 
 // accumulate the 4-spinor rebuilt from the transported two-spinor r
 template <int MU, int SG, typename R>
-__device__ __forceinline__ void reconstruct(cx<R>* a, const cx<R>* r) {
+__device__ __forceinline__ void reconstruct(cx<R>* a, const cx<R>* r, R w) {
+  // w: per-site scale of a 16-bit input (1 otherwise); each w * r term
+  // contracts with its accumulation into one fused multiply-add
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    a[c] = a[c] + r[c];
-    a[3 + c] = a[3 + c] + r[3 + c];
+    const cx<R> ru = w * r[c], rl = w * r[3 + c];
+    a[c] = a[c] + ru;
+    a[3 + c] = a[3 + c] + rl;
     if (MU == 0) {
-      a[6 + c] = a[6 + c] - sgi<SG>(r[3 + c]);
-      a[9 + c] = a[9 + c] - sgi<SG>(r[c]);
+      a[6 + c] = a[6 + c] - sgi<SG>(rl);
+      a[9 + c] = a[9 + c] - sgi<SG>(ru);
     } else if (MU == 1) {
-      a[6 + c] = a[6 + c] + sgr<SG>(r[3 + c]);
-      a[9 + c] = a[9 + c] - sgr<SG>(r[c]);
+      a[6 + c] = a[6 + c] + sgr<SG>(rl);
+      a[9 + c] = a[9 + c] - sgr<SG>(ru);
     } else if (MU == 2) {
-      a[6 + c] = a[6 + c] - sgi<SG>(r[c]);
-      a[9 + c] = a[9 + c] + sgi<SG>(r[3 + c]);
+      a[6 + c] = a[6 + c] - sgi<SG>(ru);
+      a[9 + c] = a[9 + c] + sgi<SG>(rl);
     } else {
-      a[6 + c] = a[6 + c] + sgr<SG>(r[c]);
-      a[9 + c] = a[9 + c] + sgr<SG>(r[3 + c]);
+      a[6 + c] = a[6 + c] + sgr<SG>(ru);
+      a[9 + c] = a[9 + c] + sgr<SG>(rl);
     }
   }
 }
 
 // plain load of the input spinor at site n of one slice
 // A loader fills p with the spinor at site n; for 16-bit policies it leaves
-// out the per-site scale and returns it, so the caller applies it to the
-// spin-projected half spinor (6 instead of 12 complex products).
+// out the per-site scale and returns it, so the caller folds it into the
+// accumulation of the hop's product (no separate multiplications).
 template <class P>
 struct PlainLoader {
   const typename P::C* __restrict__ d;
@@ -433,13 +446,21 @@ struct MobTLoader {
   R b, cm, cp;
   __device__ __forceinline__ float operator()(int n, cx<R>* p) const {
     cx<R> vm[12], vp[12];
-    ld_spinor<P>(cur, sc_cur, vh, n, p);
-    ld_spinor<P>(prv, sc_prv, vh, n, vm);
-    ld_spinor<P>(nxt, sc_nxt, vh, n, vp);
+    // raw values: the three per-site scales go into the coefficients
+    // (3 products per site instead of one per component)
+    ld_spinor_raw<P>(cur, vh, n, p);
+    ld_spinor_raw<P>(prv, vh, n, vm);
+    ld_spinor_raw<P>(nxt, vh, n, vp);
+    R bs = b, cms = cm, cps = cp;
+    if constexpr (P::kScaled) {
+      bs = b * sc_cur[n];
+      cms = cm * sc_prv[n];
+      cps = cp * sc_nxt[n];
+    }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) p[c] = b * p[c] + cm * vm[c];
+    for (int c = 0; c < 6; ++c) p[c] = bs * p[c] + cms * vm[c];
 #pragma unroll
-    for (int c = 6; c < 12; ++c) p[c] = b * p[c] + cp * vp[c];
+    for (int c = 6; c < 12; ++c) p[c] = bs * p[c] + cps * vp[c];
     return 1.0f;
   }
 };
@@ -524,8 +545,6 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
   {
     const R sc = (R)ld(nb.fwd[MU], psi);
     project<MU, SF>(psi, h);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) h[k] = sc * h[k];
     cx<R> m[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) m[e] = lf[e];
@@ -535,13 +554,11 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
       for (int row = 0; row < 3; ++row)
         r[3 * k + row] = row_dot(m[row * 3 + 0], m[row * 3 + 1], m[row * 3 + 2], h[3 * k + 0],
                                  h[3 * k + 1], h[3 * k + 2]);
-    reconstruct<MU, SF>(acc, r);
+    reconstruct<MU, SF>(acc, r, sc);
   }
   {
     const R sc = (R)ld(nb.bwd[MU], psi);
     project<MU, -SF>(psi, h);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) h[k] = sc * h[k];
     cx<R> m[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) m[e] = lb[e];
@@ -551,7 +568,7 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
       for (int row = 0; row < 3; ++row)
         r[3 * k + row] = row_dotc(m[0 * 3 + row], m[1 * 3 + row], m[2 * 3 + row], h[3 * k + 0],
                                   h[3 * k + 1], h[3 * k + 2]);
-    reconstruct<MU, -SF>(acc, r);
+    reconstruct<MU, -SF>(acc, r, sc);
   }
 }
 
@@ -611,6 +628,18 @@ struct CgEpi {
   const double* scal;
 };
 
+// PX = NoX: the epilogue updates r only; x += alpha p is left to the pass
+// that forms the next direction (k_cg_xp), which reads p anyway.  This keeps
+// the x traffic and its arithmetic out of the hop kernel.
+struct NoX {
+  typedef float R;
+  typedef float S;
+  typedef cx<float> C;
+  static constexpr bool kScaled = false;
+  static constexpr int kCPC = 1;
+  static constexpr int kLinkElems = 9;
+};
+
 template <class P, class PX>
 __device__ __forceinline__ double cg_epilogue(const CgEpi<P, PX>& cg, const Geom& g, int s,
                                               int i, const cx<typename P::R>* ap) {
@@ -620,17 +649,21 @@ __device__ __forceinline__ double cg_epilogue(const CgEpi<P, PX>& cg, const Geom
   const double a = d != 0.0 ? rr / d : 0.0;
   const size_t sl = (size_t)s * nch<P>() * g.vh, slx = (size_t)s * nch<PX>() * g.vh;
   const size_t ss = (size_t)s * g.vh;
-  cx<R> r[12], p[12];
-  cx<RX> x[12];
+  cx<R> r[12];
   ld_spinor<P>(cg.r.d + sl, cg.r.sc + ss, g.vh, i, r);
-  ld_spinor<P>(cg.p.d + sl, cg.p.sc + ss, g.vh, i, p);
-  ld_spinor<PX>(cg.x.d + slx, cg.x.sc + ss, g.vh, i, x);
+  if constexpr (!std::is_same<PX, NoX>::value) {
+    cx<R> p[12];
+    cx<RX> x[12];
+    ld_spinor<P>(cg.p.d + sl, cg.p.sc + ss, g.vh, i, p);
+    ld_spinor<PX>(cg.x.d + slx, cg.x.sc + ss, g.vh, i, x);
 #pragma unroll
-  for (int c = 0; c < 12; ++c) {
-    x[c] = cx<RX>{(RX)(a * p[c].re) + x[c].re, (RX)(a * p[c].im) + x[c].im};
-    r[c] = cx<R>{(R)(-a * ap[c].re) + r[c].re, (R)(-a * ap[c].im) + r[c].im};
+    for (int c = 0; c < 12; ++c)
+      x[c] = cx<RX>{(RX)(a * p[c].re) + x[c].re, (RX)(a * p[c].im) + x[c].im};
+    st_site<PX>(cg.x.d + slx, cg.x.sc + ss, g.vh, i, x);
   }
-  st_site<PX>(cg.x.d + slx, cg.x.sc + ss, g.vh, i, x);
+#pragma unroll
+  for (int c = 0; c < 12; ++c)
+    r[c] = cx<R>{(R)(-a * ap[c].re) + r[c].re, (R)(-a * ap[c].im) + r[c].im};
   st_site<P>(cg.r.d + sl, cg.r.sc + ss, g.vh, i, r);
   return norm12(r);
 }
@@ -1046,6 +1079,33 @@ k_axpby(Fld<P> y, CFld<P> x, double a, double b, Geom g) {
   }
 }
 
+// x += a p, then (UPDATE_P) p = r + b p: the vector updates of a CG step
+// whose hop epilogue updated r only (PX = NoX there); x in its own policy.
+template <class P, class PX, bool UPDATE_P>
+__global__ void __launch_bounds__(BLAS_THREADS)
+k_cg_xp(Fld<PX> x, Fld<P> p, CFld<P> r, double a, double b, Geom g) {
+  typedef typename P::R R;
+  typedef typename PX::R RX;
+  const R br = (R)b;
+  DWFK_SITE_LOOP(k, g) {
+    cx<R> pv[12];
+    cx<RX> xv[12];
+    ld_site<P>(pv, CFld<P>{p.d, p.sc}, g, k);
+    ld_site<PX>(xv, CFld<PX>{x.d, x.sc}, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c)
+      xv[c] = cx<RX>{(RX)(a * pv[c].re) + xv[c].re, (RX)(a * pv[c].im) + xv[c].im};
+    st_site_k<PX>(x, g, k, xv);
+    if (UPDATE_P) {
+      cx<R> rv[12];
+      ld_site<P>(rv, r, g, k);
+#pragma unroll
+      for (int c = 0; c < 12; ++c) pv[c] = rv[c] + br * pv[c];
+      st_site_k<P>(p, g, k, pv);
+    }
+  }
+}
+
 // z = a x + y, partial |z|^2 optional
 template <class P, bool NORM>
 __global__ void __launch_bounds__(BLAS_THREADS)
@@ -1171,6 +1231,143 @@ k_field_to_cps(double* __restrict__ cps, CFld<P> f, Geom g) {
       dst[2 * c] = (double)v[c].re;
       dst[2 * c + 1] = (double)v[c].im;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host formats of whole fermion fields (FdwfBase::Fconvert, convert.C:639-831;
+// Ffour2five / Ffive2four, f_dwf_base.C:961-1168).  `canon` is one 4D slice
+// in CANONICAL order: site x + lx (y + ly (z + lz t)), 24 reals.  The 5D site
+// (x, s) lives in the half `first` or `second` of a full vector:
+//   DWF    (convert.C:728-760): first = 5D-odd half ((x+y+z+t+s) odd);
+//   Mobius (convert.C:797-831): first = 4D-even half, every slice.
+// up / lo select the upper (reals 0..11) and lower (12..23) spin pair.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool site_in_first(int p4, int s, bool dwf) {
+  return dwf ? ((p4 + s) & 1) == 1 : p4 == 0;
+}
+__device__ __forceinline__ void canon_coords(const Geom& g, size_t k, int* c) {
+  c[0] = (int)(k % g.lx);
+  size_t r = k / g.lx;
+  c[1] = (int)(r % g.ly);
+  r /= g.ly;
+  c[2] = (int)(r % g.lz);
+  c[3] = (int)(r / g.lz);
+}
+
+// slice s of (first, second) = the spin pairs of canon selected by up / lo,
+// zero elsewhere; canon == nullptr writes zeros
+template <class P>
+__global__ void __launch_bounds__(256)
+k_canon_to_halves(Fld<P> first, Fld<P> second, const double* __restrict__ canon, Geom g, int s,
+                  int dwf, int up, int lo) {
+  typedef typename P::R R;
+  const size_t v4 = (size_t)2 * g.vh;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < v4;
+       k += (size_t)gridDim.x * blockDim.x) {
+    int c[4];
+    canon_coords(g, k, c);
+    const int p4 = (c[0] + c[1] + c[2] + c[3]) & 1;
+    const int i = site_index(g, c[0] >> 1, c[1], c[2], c[3]);
+    cx<R> v[12];
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      const bool take = canon && (q < 6 ? up : lo);
+      v[q] = take ? cx<R>{(R)canon[k * 24 + 2 * q], (R)canon[k * 24 + 2 * q + 1]} : cx<R>{0, 0};
+    }
+    st_site_k<P>(site_in_first(p4, s, dwf) ? first : second, g, (size_t)s * g.vh + i, v);
+  }
+}
+
+// the spin pairs selected by up / lo of canon = slice s of (first, second);
+// the other reals of canon are left as they are
+template <class P>
+__global__ void __launch_bounds__(256)
+k_halves_to_canon(double* __restrict__ canon, CFld<P> first, CFld<P> second, Geom g, int s,
+                  int dwf, int up, int lo) {
+  typedef typename P::R R;
+  const size_t v4 = (size_t)2 * g.vh;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < v4;
+       k += (size_t)gridDim.x * blockDim.x) {
+    int c[4];
+    canon_coords(g, k, c);
+    const int p4 = (c[0] + c[1] + c[2] + c[3]) & 1;
+    const int i = site_index(g, c[0] >> 1, c[1], c[2], c[3]);
+    cx<R> v[12];
+    ld_site<P>(v, site_in_first(p4, s, dwf) ? first : second, g, (size_t)s * g.vh + i);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      if (q < 6 ? up : lo) {
+        canon[k * 24 + 2 * q] = (double)v[q].re;
+        canon[k * 24 + 2 * q + 1] = (double)v[q].im;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gauge observables of the FP64 device links (ReadLatticeParallel::
+// CheckPlaqLinktrace, ReadLatticePar.C:396-450; Lattice::SumReTrPlaqNode,
+// lattice_base.C:2878-2905): per block, the sum over its sites of
+// ReTr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag for mu < nu, and of
+// ReTr U_mu(x).  The anti-periodic sign carried by the device links cancels
+// in every plaquette (a boundary link of direction mu enters it twice or
+// not at all); the link trace divides it out.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_link64(cx<double>* m, const cx<double>* __restrict__ U,
+                                            const Geom& g, int mu, const int* c) {
+  const int par = (c[0] + c[1] + c[2] + c[3]) & 1;
+  const int i = site_index(g, c[0] >> 1, c[1], c[2], c[3]);
+  const cx<double>* src = U + (size_t)((mu * 2 + par) * 9) * g.vh + i;
+#pragma unroll
+  for (int e = 0; e < 9; ++e) m[e] = src[(size_t)e * g.vh];
+}
+__device__ __forceinline__ void mat_mul3(cx<double>* o, const cx<double>* a, const cx<double>* b) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      o[r * 3 + q] = cmul(a[r * 3 + 0], b[0 * 3 + q]) + cmul(a[r * 3 + 1], b[1 * 3 + q]) +
+                     cmul(a[r * 3 + 2], b[2 * 3 + q]);
+}
+
+__global__ void __launch_bounds__(256)
+k_plaquette(const cx<double>* __restrict__ U, Geom g, double* __restrict__ partial) {
+  __shared__ double sh[16];
+  const size_t v4 = (size_t)2 * g.vh;
+  const int L[4] = {g.lx, g.ly, g.lz, g.lt};
+  double plaq = 0.0, tr = 0.0;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < v4;
+       k += (size_t)gridDim.x * blockDim.x) {
+    int c[4];
+    canon_coords(g, k, c);
+    for (int mu = 0; mu < 4; ++mu) {
+      cx<double> umu[9];
+      load_link64(umu, U, g, mu, c);
+      const bool neg = ((g.bcmask >> mu) & 1) && c[mu] == L[mu] - 1;
+      tr += (neg ? -1.0 : 1.0) * (umu[0].re + umu[4].re + umu[8].re);
+      for (int nu = mu + 1; nu < 4; ++nu) {
+        int cm[4] = {c[0], c[1], c[2], c[3]}, cn[4] = {c[0], c[1], c[2], c[3]};
+        cm[mu] = cm[mu] + 1 == L[mu] ? 0 : cm[mu] + 1;
+        cn[nu] = cn[nu] + 1 == L[nu] ? 0 : cn[nu] + 1;
+        cx<double> unu[9], a[9], b[9], t[9];
+        load_link64(t, U, g, nu, cm);  // U_nu(x + mu)
+        mat_mul3(a, umu, t);
+        load_link64(unu, U, g, nu, c);
+        load_link64(t, U, g, mu, cn);  // U_mu(x + nu)
+        mat_mul3(b, unu, t);
+        // ReTr(a b^dag) = sum_ij Re(a_ij conj(b_ij))
+#pragma unroll
+        for (int e = 0; e < 9; ++e) plaq += a[e].re * b[e].re + a[e].im * b[e].im;
+      }
+    }
+  }
+  const double tp = block_sum(plaq, sh);
+  __syncthreads();
+  const double tt = block_sum(tr, sh);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = tp;
+    partial[gridDim.x + blockIdx.x] = tt;
   }
 }
 

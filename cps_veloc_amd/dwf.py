@@ -24,7 +24,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 # CPS_DWF_GPU_LIB selects another build of the same library (A/B timing)
 LIB_PATH = os.environ.get("CPS_DWF_GPU_LIB", os.path.join(_HERE, "libcps_dwf_gpu.so"))
 
-OK, BREAKDOWN, MAXITER, BAD_ARG, HIP_ERROR = 0, 1, 2, 3, 4
+OK, BREAKDOWN, MAXITER, BAD_ARG, HIP_ERROR, IO_ERROR, CHECKSUM_ERROR = 0, 1, 2, 3, 4, 5, 6
 DWF, MOBIUS = 0, 1
 F64, F32, BF16, I16 = 0, 1, 2, 3
 _PREC = {"f64": F64, "f32": F32, "bf16": BF16, "i16": I16}
@@ -45,6 +45,8 @@ EXPORTS = (
     "dwfgpu_ptc_get_weights", "dwfgpu_ptc_apply", "dwfgpu_ptc_loss_grad", "dwfgpu_ptc_train",
     "dwfgpu_invcg_ptc_dev", "dwfgpu_minvcg", "dwfgpu_minvcg_dev",
     "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
+    "dwfgpu_fermion_from_canonical", "dwfgpu_fermion_to_canonical", "dwfgpu_four2five",
+    "dwfgpu_five2four", "dwfgpu_matinv_dev", "dwfgpu_nersc_read", "dwfgpu_plaquette",
 )
 # every entry point of include/dwf_synth.h
 SYNTH_EXPORTS = ("dwfsynth_gauge", "dwfsynth_gaussian")
@@ -62,6 +64,13 @@ class PtcTrainArgC(C.Structure):
                 ("random_pairs", C.c_int), ("stride", C.c_int), ("epochs", C.c_int),
                 ("batch", C.c_int),
                 ("lr", C.c_double), ("seed", C.c_ulonglong)]
+
+
+class NerscHeaderC(C.Structure):
+    _fields_ = [("dims", C.c_int * 4), ("recon_row_3", C.c_int), ("fp_bytes", C.c_int),
+                ("big_endian", C.c_int), ("checksum", C.c_uint), ("checksum_data", C.c_uint),
+                ("plaquette", C.c_double), ("link_trace", C.c_double),
+                ("data_start", C.c_longlong)]
 
 
 class CgArgC(C.Structure):
@@ -149,6 +158,15 @@ def lib():
         L.dwfgpu_matpcdagmatpc_cost.argtypes = [H, C.c_int, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double)]
         L.dwfgpu_time_copy.argtypes = [H, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.dwfgpu_fermion_from_canonical.argtypes = [H, H, H, _dp]
+        L.dwfgpu_fermion_to_canonical.argtypes = [H, _dp, H, H]
+        L.dwfgpu_four2five.argtypes = [H, H, H, _dp, C.c_int, C.c_int]
+        L.dwfgpu_five2four.argtypes = [H, _dp, H, H, C.c_int, C.c_int]
+        L.dwfgpu_matinv_dev.argtypes = [H, C.POINTER(CgArgC), H, H, H, H,
+                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.dwfgpu_nersc_read.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t,
+                                        C.POINTER(NerscHeaderC)]
+        L.dwfgpu_plaquette.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.dwfsynth_gauge.argtypes = [_dp, C.c_int * 4, C.c_double, C.c_ulonglong]
         L.dwfsynth_gauge.restype = None
         L.dwfsynth_gaussian.argtypes = [_dp, C.c_size_t, C.c_ulonglong]
@@ -359,7 +377,8 @@ class _DiracOpGpu:
         al = None if alpha is None else np.ascontiguousarray(alpha, np.float64)
         if psi is None:
             psi = np.zeros((ns if multi else 1) * self.n)
-        psi = np.ascontiguousarray(psi, np.float64).reshape(-1)
+        # the result goes into a copy: the caller's starting vector is kept
+        psi = np.array(psi, dtype=np.float64, copy=True).reshape(-1)
         its = np.zeros(ns, dtype=np.int32)
         k = C.c_int()
         st = _chk(lib().dwfgpu_minvcg(self.h, mass, shifts, ns, psi, _arr(chi), -1.0, rs,
@@ -446,6 +465,98 @@ class _DiracOpGpu:
         p = _PREC[precision or self.precision]
         _chk(lib().dwfgpu_matpcdagmatpc_cost(self.h, p, C.byref(b), C.byref(f)), "cost")
         return b.value, f.value
+
+    # --- host formats (convert.C:639-831, f_dwf_base.C:961-1168) ----------
+    def _v4(self):
+        return int(np.prod(self.dims))
+
+    def fermion_from_canonical(self, canon):
+        """CANONICAL 5D fermion [s][x,y,z,t][24] -> (first, second) device halves
+        of the MatInv full vector (Fconvert CANONICAL -> WILSON / EOPREC_EE)."""
+        canon = _arr(canon).reshape(-1)
+        if canon.size != self.ls * self._v4() * 24:
+            raise ValueError("canonical field must hold Ls*V4*24 reals")
+        a, b = self.vector(), self.vector()
+        _chk(lib().dwfgpu_fermion_from_canonical(self.h, a.h, b.h, canon),
+             "fermion_from_canonical")
+        return a, b
+
+    def fermion_to_canonical(self, first: DeviceVector, second: DeviceVector):
+        out = np.empty(self.ls * self._v4() * 24)
+        _chk(lib().dwfgpu_fermion_to_canonical(self.h, out, first.h, second.h),
+             "fermion_to_canonical")
+        return out
+
+    def Ffour2five(self, four, s_u, s_l):
+        """FdwfBase::Ffour2five (Ncb 2) into device halves (first, second)."""
+        four = _arr(four).reshape(-1)
+        if four.size != self._v4() * 24:
+            raise ValueError("4D field must hold V4*24 reals")
+        a, b = self.vector(), self.vector()
+        _chk(lib().dwfgpu_four2five(self.h, a.h, b.h, four, s_u, s_l), "Ffour2five")
+        return a, b
+
+    def Ffive2four(self, first: DeviceVector, second: DeviceVector, s_u, s_l):
+        out = np.empty(self._v4() * 24)
+        _chk(lib().dwfgpu_five2four(self.h, out, first.h, second.h, s_u, s_l), "Ffive2four")
+        return out
+
+    def MatInv_dev(self, out_first, out_second, in_first, in_second, cg_arg: CgArg):
+        """MatInv on device halves; returns (iters, true_res, status)."""
+        tr, it = C.c_double(), C.c_int()
+        a = cg_arg.c()
+        st = _chk(lib().dwfgpu_matinv_dev(self.h, C.byref(a), out_first.h, out_second.h,
+                                          in_first.h, in_second.h, C.byref(tr), C.byref(it)),
+                  "MatInv_dev", allow=(OK, MAXITER))
+        return it.value, tr.value, st
+
+    def plaquette(self):
+        """(SumReTrPlaq / (18 V), sum ReTr U / (12 V)) of the handle's links."""
+        p, t = C.c_double(), C.c_double()
+        _chk(lib().dwfgpu_plaquette(self.h, C.byref(p), C.byref(t)), "plaquette")
+        return p.value, t.value
+
+
+@dataclass
+class NerscHeader:
+    dims: tuple
+    recon_row_3: int
+    fp_bytes: int
+    big_endian: int
+    checksum: int
+    checksum_data: int
+    plaquette: float
+    link_trace: float
+    data_start: int
+
+
+def read_nersc(path, with_links=True):
+    """ReadLatticeParallel::read steps 0-2 (ReadLatticePar.C:26-175): returns
+    (links CANONICAL [V4*4*18] or None, NerscHeader).  Raises DwfGpuError
+    on a missing/short file or a checksum mismatch."""
+    h = NerscHeaderC()
+    _chk(lib().dwfgpu_nersc_read(os.fsencode(path), None, 0, C.byref(h)), "nersc_read")
+    links = None
+    if with_links:
+        v4 = int(np.prod(list(h.dims)))
+        links = np.empty(v4 * 4 * 18)
+        _chk(lib().dwfgpu_nersc_read(os.fsencode(path), links.ctypes.data_as(C.c_void_p),
+                                     links.size, C.byref(h)), "nersc_read")
+    return links, NerscHeader(tuple(h.dims), h.recon_row_3, h.fp_bytes, h.big_endian,
+                              h.checksum, h.checksum_data, h.plaquette, h.link_trace,
+                              h.data_start)
+
+
+def check_plaquette(op: "_DiracOpGpu", hdr: NerscHeader, chk_prec):
+    """ReadLatticeParallel::CheckPlaqLinktrace (ReadLatticePar.C:396-450):
+    the plaquette of the loaded links against the header's, relative
+    deviation at most chk_prec (the link trace is reported, not checked)."""
+    plaq, tr = op.plaquette()
+    dev = abs((plaq - hdr.plaquette) / plaq)
+    if dev > chk_prec:
+        raise ValueError(f"plaquette {plaq:.8e} differs from header {hdr.plaquette:.8e} "
+                         f"(rel. dev. {dev:.3e} > {chk_prec})")
+    return plaq, tr
 
 
 class DiracOpDwfGpu(_DiracOpGpu):
