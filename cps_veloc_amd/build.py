@@ -8,8 +8,10 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcps_dwf_gpu.so")
-SOURCES = [os.path.join(HERE, "csrc", "dwf_gpu.hip"), os.path.join(HERE, "csrc", "synth.cpp")]
+SOURCES = [os.path.join(HERE, "csrc", "dwf_gpu.hip"), os.path.join(HERE, "csrc", "synth.cpp"),
+           os.path.join(HERE, "csrc", "nersc.cpp")]
 HEADERS = [os.path.join(HERE, "csrc", "dwf_kernels.hpp"),
+           os.path.join(HERE, "csrc", "ptc_kernels.hpp"),
            os.path.join(ROOT, "include", "dwf_gpu.h"),
            os.path.join(ROOT, "include", "roofline.h"),
            os.path.join(ROOT, "include", "dwf_synth.h")]

@@ -29,6 +29,8 @@ extern "C" {
 #define DWFGPU_MAXITER 2   /* CG hit max_num_iter (CPS: VRB.Warn, inv_cg.C:392-396)  */
 #define DWFGPU_BAD_ARG 3
 #define DWFGPU_HIP_ERROR 4
+#define DWFGPU_IO_ERROR 5       /* file missing, short, or header unreadable (ERR.FileR) */
+#define DWFGPU_CHECKSUM_ERROR 6 /* NERSC data checksum differs from the header's        */
 
 /* actions */
 #define DWFGPU_DWF 0    /* DiracOpDwf    (src/util/dirac_op/d_op_dwf/d_op_dwf.C)       */
@@ -246,6 +248,65 @@ int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, int precision, double *bytes, double *
  * down to 16) between two fresh device buffers `reps` times; *gbs receives
  * (read + write bytes) / mean kernel time.                                   */
 int dwfgpu_time_copy(dwfgpu_t h, size_t bytes, int reps, double *gbs);
+
+/* ---- (f2) host formats of whole fermion fields and gauge files --------- */
+/* Full 5D fermion in CANONICAL order, FdwfBase::Fconvert(CANONICAL <->
+ * WILSON / DWF_4D_EOPREC_EE) (src/util/lattice/convert/convert.C:639-831):
+ * canon = Ls slices of V4 sites, site x + lx (y + ly (z + lz t)), 24 reals
+ * each (Ls*V4*24 reals).  first / second are the device halves of the full
+ * vector in MatInv order: DWF [5D-odd | 5D-even], Mobius [4D-even | 4D-odd].
+ * Vectors are the handle's own (dwfgpu_vec_create).                        */
+int dwfgpu_fermion_from_canonical(dwfgpu_t h, dwfgpu_vec_t first, dwfgpu_vec_t second,
+                                  const double *canon);
+int dwfgpu_fermion_to_canonical(dwfgpu_t h, double *canon, dwfgpu_vec_t first,
+                                dwfgpu_vec_t second);
+/* FdwfBase::Ffour2five(five, four, s_u, s_l, Ncb = 2)
+ * (src/util/lattice/f_dwf_base/f_dwf_base.C:961-1043) fused with the
+ * conversion above: four = one CANONICAL 4D fermion (V4*24 reals); the 5D
+ * field is zero except the upper spin pair (reals 0..11) at slice s_u and the
+ * lower pair (12..23) at slice s_l.  0 <= s_u, s_l < Ls (one node holds
+ * every slice), DWFGPU_BAD_ARG otherwise.                                   */
+int dwfgpu_four2five(dwfgpu_t h, dwfgpu_vec_t first, dwfgpu_vec_t second, const double *four,
+                     int s_u, int s_l);
+/* FdwfBase::Ffive2four(four, five, s_u, s_l, Ncb = 2) (f_dwf_base.C:1064-1168):
+ * four = upper pair of slice s_u and lower pair of slice s_l, CANONICAL.     */
+int dwfgpu_five2four(dwfgpu_t h, double *four, dwfgpu_vec_t first, dwfgpu_vec_t second,
+                     int s_u, int s_l);
+/* DiracOpDwf::MatInv / DiracOpMobius::MatInv (d_op_dwf.C:316-507,
+ * d_op_mobius.C:407-696) on device halves; the inputs are preserved
+ * (PRESERVE_YES).  out and in must be distinct vectors.                     */
+int dwfgpu_matinv_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t out_first,
+                      dwfgpu_vec_t out_second, dwfgpu_vec_t in_first, dwfgpu_vec_t in_second,
+                      double *true_res, int *iters);
+
+/* Header of a NERSC gauge file (LatticeHeader::read, src/util/qcdio/comsrc/
+ * latheader.C:267-312).                                                      */
+typedef struct {
+  int dims[4];                /* DIMENSION_1..4                                 */
+  int recon_row_3;            /* DATATYPE 4D_SU3_GAUGE: two rows stored         */
+  int fp_bytes;               /* 4 or 8 (FLOATING_POINT)                        */
+  int big_endian;             /* byte order of FLOATING_POINT                   */
+  unsigned int checksum;      /* CHECKSUM of the header                         */
+  unsigned int checksum_data; /* the same 32-bit sum over the data read         */
+  double plaquette;           /* PLAQUETTE (read as single precision, as CPS)   */
+  double link_trace;          /* LINK_TRACE                                     */
+  long long data_start;       /* byte offset of the data                        */
+} dwfgpu_nersc_header;
+/* ReadLatticeParallel::read steps 0-2 (src/util/qcdio/comsrc/
+ * ReadLatticePar.C:26-175): header, data in file order t,z,y,x / mu / row-
+ * major 3x3 (the CANONICAL order), the FPConv 32-bit checksum, and the third
+ * row conj(row0 x row1) when two rows are stored.  gauge = 4*V4*18 reals
+ * (n_reals), or NULL to read the header only.  DWFGPU_IO_ERROR: no file,
+ * no END_HEADER, bad dimensions, unknown FLOATING_POINT or short data;
+ * DWFGPU_CHECKSUM_ERROR: the links are filled but the sums differ.  Host
+ * code (file I/O); the plaquette check is dwfgpu_plaquette.                 */
+int dwfgpu_nersc_read(const char *path, double *gauge, size_t n_reals,
+                      dwfgpu_nersc_header *hdr);
+/* Lattice::SumReTrPlaq / (18 V) and sum ReTr U / (12 V) of the handle's
+ * links without the boundary signs, as ReadLatticeParallel::
+ * CheckPlaqLinktrace compares them with the header (ReadLatticePar.C:396-450).
+ * Either pointer may be NULL.                                                */
+int dwfgpu_plaquette(dwfgpu_t h, double *plaq, double *link_trace);
 
 #ifdef __cplusplus
 }

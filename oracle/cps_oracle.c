@@ -115,6 +115,107 @@ void orc_fermion_insert_cb(double *canon, const double *half, const int L[4], in
 }
 
 /* ------------------------------------------------------------------------ */
+/* f2: whole 5D fermion fields, CANONICAL <-> the full vector of MatInv      */
+/* ------------------------------------------------------------------------ */
+/* One CANONICAL 4D slice -> WILSON order, odd sites first
+ * (convert_func.C:452-462: offset vol/2 * ((x+y+z+t+1)%2) + cb index).     */
+static void canon_slice_to_wilson(double *wil, const double *canon, const int L[4]) {
+  const int lx = L[0], ly = L[1], lz = L[2], lt = L[3];
+  const size_t vh = (size_t)lx * ly * lz * lt / 2;
+  for (int t = 0; t < lt; ++t)
+    for (int z = 0; z < lz; ++z)
+      for (int y = 0; y < ly; ++y)
+        for (int x = 0; x < lx; ++x) {
+          const size_t site = x + (size_t)lx * (y + (size_t)ly * (z + (size_t)lz * t));
+          const size_t cbs = x / 2 + (size_t)(lx / 2) * (y + (size_t)ly * (z + (size_t)lz * t));
+          const size_t w = vh * (size_t)((x + y + z + t + 1) % 2) + cbs;
+          memcpy(wil + w * SPINOR, canon + site * SPINOR, SPINOR * sizeof(double));
+        }
+}
+
+/* FdwfBase::Fconvert CANONICAL -> WILSON (DWF, convert.C:704-757) or
+ * -> DWF_4D_EOPREC_EE (Mobius, convert.C:797-831): every slice to WILSON
+ * order, then the halves of the slices gathered into [first | second].     */
+void orc_fermion5d_canonical_to_full(double *full, const double *canon, const orc_params *p) {
+  const size_t v4 = (size_t)p->L[0] * p->L[1] * p->L[2] * p->L[3];
+  const size_t hs = v4 / 2 * SPINOR, stride = 2 * hs;
+  const int ls = p->ls;
+  double *tmp = (double *)malloc((size_t)ls * stride * sizeof(double));
+  for (int s = 0; s < ls; ++s) canon_slice_to_wilson(tmp + s * stride, canon + s * stride, p->L);
+  double *out = full;
+  if (p->action == 0) {
+    /* odd part: slice i takes the half of 4D parity (i+1)%2; the source
+     * pointer advances by (2 parity + 1) halves (convert.C:724-739)        */
+    const double *src = tmp;
+    for (int i = 0; i < ls; ++i) {
+      const int parity = (i + 1) % 2;
+      memcpy(out, src, hs * sizeof(double));
+      out += hs;
+      src += (size_t)(2 * parity + 1) * hs;
+    }
+    /* even part, starting one half in (convert.C:741-756)                  */
+    src = tmp + hs;
+    for (int i = 0; i < ls; ++i) {
+      const int parity = i % 2;
+      memcpy(out, src, hs * sizeof(double));
+      out += hs;
+      src += (size_t)(2 * parity + 1) * hs;
+    }
+  } else {
+    /* even part (second WILSON half of each slice), then odd part          */
+    for (int i = 0; i < ls; ++i, out += hs)
+      memcpy(out, tmp + i * stride + hs, hs * sizeof(double));
+    for (int i = 0; i < ls; ++i, out += hs) memcpy(out, tmp + i * stride, hs * sizeof(double));
+  }
+  free(tmp);
+}
+
+/* the inverse permutation: [first | second] -> CANONICAL                    */
+void orc_fermion5d_full_to_canonical(double *canon, const double *full, const orc_params *p) {
+  const size_t v4 = (size_t)p->L[0] * p->L[1] * p->L[2] * p->L[3];
+  const size_t hs = v4 / 2 * SPINOR, stride = 2 * hs;
+  const int ls = p->ls;
+  const double *first = full, *second = full + (size_t)ls * hs;
+  for (int s = 0; s < ls; ++s) {
+    const double *even, *odd;
+    if (p->action == 0) {
+      /* 5D-odd half holds 4D parity (s+1)%2 at slice s, 5D-even parity s%2 */
+      odd = (s % 2 == 0) ? first + (size_t)s * hs : second + (size_t)s * hs;
+      even = (s % 2 == 0) ? second + (size_t)s * hs : first + (size_t)s * hs;
+    } else {
+      even = first + (size_t)s * hs;
+      odd = second + (size_t)s * hs;
+    }
+    orc_fermion_insert_cb(canon + s * stride, even, p->L, 0);
+    orc_fermion_insert_cb(canon + s * stride, odd, p->L, 1);
+  }
+}
+
+/* FdwfBase::Ffour2five(five, four, s_u, s_l, Ncb = 2), one node in s
+ * (f_dwf_base.C:961-1043): zero, then the upper spin pair of slice s_u and
+ * the lower pair of slice s_l from the 4D field.  CANONICAL order.          */
+void orc_four2five(double *five, const double *four, int s_u, int s_l, const orc_params *p) {
+  const size_t v4 = (size_t)p->L[0] * p->L[1] * p->L[2] * p->L[3];
+  const size_t ls_stride = SPINOR * v4;
+  memset(five, 0, (size_t)p->ls * ls_stride * sizeof(double));
+  for (size_t x = 0; x < v4; ++x)
+    for (int i = 0; i < 12; ++i) five[(size_t)s_u * ls_stride + x * SPINOR + i] = four[x * SPINOR + i];
+  for (size_t x = 0; x < v4; ++x)
+    for (int i = 12; i < 24; ++i) five[(size_t)s_l * ls_stride + x * SPINOR + i] = four[x * SPINOR + i];
+}
+
+/* FdwfBase::Ffive2four(four, five, s_u, s_l, Ncb = 2) (f_dwf_base.C:1064-1168) */
+void orc_five2four(double *four, const double *five, int s_u, int s_l, const orc_params *p) {
+  const size_t v4 = (size_t)p->L[0] * p->L[1] * p->L[2] * p->L[3];
+  const size_t ls_stride = SPINOR * v4;
+  memset(four, 0, v4 * SPINOR * sizeof(double));
+  for (size_t x = 0; x < v4; ++x)
+    for (int i = 0; i < 12; ++i) four[x * SPINOR + i] = five[(size_t)s_u * ls_stride + x * SPINOR + i];
+  for (size_t x = 0; x < v4; ++x)
+    for (int i = 12; i < 24; ++i) four[x * SPINOR + i] = five[(size_t)s_l * ls_stride + x * SPINOR + i];
+}
+
+/* ------------------------------------------------------------------------ */
 /* a1: wilson_dslash  (d_op_wilson/noarch/wilson_dslash.C:64-167, 173-517)    */
 /* ------------------------------------------------------------------------ */
 /* (1 -/+ gamma_mu) psi, unprojected 4-spinor, sign = -1 for "minus" (mp=0),

@@ -42,6 +42,15 @@
  *                           eigenvectors; solution vs dense solve; MAXITER and
  *                           converged-guess edges
  *   orc_matinv              full textbook system solved for both actions
+ *   orc_fermion5d_canonical_to_full / _full_to_canonical
+ *                           against the slice-wise composition of
+ *                           orc_fermion_extract_cb with the parity rule of
+ *                           each action; round trip; every slice of DWF's
+ *                           5D-odd half holds sites with x+y+z+t+s odd
+ *   orc_four2five, orc_five2four
+ *                           five2four(four2five(f)) = f for s_u != s_l and
+ *                           s_u == s_l; zero outside the two chiral slices;
+ *                           chirality of the slices (gamma5 = +1 at s_u)
  */
 #ifndef CPS_ORACLE_H
 #define CPS_ORACLE_H
@@ -70,6 +79,16 @@ void orc_gauge_canonical_to_wilson(double *u_wilson, const double *u_canon,
 /* One CANONICAL 4D fermion slice [site][24] -> half-lattice of parity p. */
 void orc_fermion_extract_cb(double *half, const double *canon, const int L[4], int p);
 void orc_fermion_insert_cb(double *canon, const double *half, const int L[4], int p);
+
+/* ---- f2: whole 5D fermion fields (CANONICAL: [s][4D site][24]) -------- */
+/* Fconvert CANONICAL -> the MatInv full vector [first | second]: DWF
+ * [5D-odd | 5D-even] (convert.C:704-757), Mobius [4D-even | 4D-odd]
+ * (convert.C:797-831); and back.                                            */
+void orc_fermion5d_canonical_to_full(double *full, const double *canon, const orc_params *p);
+void orc_fermion5d_full_to_canonical(double *canon, const double *full, const orc_params *p);
+/* Ffour2five / Ffive2four with Ncb = 2 (f_dwf_base.C:961-1168).             */
+void orc_four2five(double *five, const double *four, int s_u, int s_l, const orc_params *p);
+void orc_five2four(double *four, const double *five, int s_u, int s_l, const orc_params *p);
 
 /* ---- a1: Wilson 4D hopping term ------------------------------------- */
 void orc_wilson_dslash(double *chi, const double *u, const double *psi,

@@ -57,6 +57,12 @@ def lib():
         L.orc_minvcg.restype = C.c_int
         L.orc_minvcg.argtypes = [_dp, _dp, C.c_double, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
                                  _ip, _dp, C.c_double, P]
+        L.orc_fermion_extract_cb.argtypes = [_dp, _dp, C.c_int * 4, C.c_int]
+        L.orc_fermion_insert_cb.argtypes = [_dp, _dp, C.c_int * 4, C.c_int]
+        L.orc_fermion5d_canonical_to_full.argtypes = [_dp, _dp, P]
+        L.orc_fermion5d_full_to_canonical.argtypes = [_dp, _dp, P]
+        L.orc_four2five.argtypes = [_dp, _dp, C.c_int, C.c_int, P]
+        L.orc_five2four.argtypes = [_dp, _dp, C.c_int, C.c_int, P]
         L.orc_norm_sq.restype = C.c_double
         L.orc_norm_sq.argtypes = [_dp, C.c_size_t]
         _lib = L
@@ -157,6 +163,31 @@ class Oracle:
                              max_iter, its, self.u, mass, C.byref(self.p))
         return psi.reshape(ns, self.n) if multi else psi, k, rs, its
 
+    # -- f2: whole 5D fermion fields (CANONICAL [s][4D site][24]) ----------
+    def canonical_to_full(self, canon):
+        full = np.zeros(2 * self.n)
+        lib().orc_fermion5d_canonical_to_full(full, np.ascontiguousarray(canon, np.float64),
+                                              C.byref(self.p))
+        return full
+
+    def full_to_canonical(self, full):
+        canon = np.zeros(2 * self.n)
+        lib().orc_fermion5d_full_to_canonical(canon, np.ascontiguousarray(full, np.float64),
+                                              C.byref(self.p))
+        return canon
+
+    def four2five(self, four, s_u, s_l):
+        five = np.zeros(2 * self.n)
+        lib().orc_four2five(five, np.ascontiguousarray(four, np.float64), s_u, s_l,
+                            C.byref(self.p))
+        return five
+
+    def five2four(self, five, s_u, s_l):
+        four = np.zeros(2 * self.n // self.ls)
+        lib().orc_five2four(four, np.ascontiguousarray(five, np.float64), s_u, s_l,
+                            C.byref(self.p))
+        return four
+
     def kappas(self):
         p = self.p
         if self.action == "dwf":
@@ -164,3 +195,34 @@ class Oracle:
         kb = 1.0 / (2 * (p.mob_b * (4 - p.m5) + p.a5inv))
         kc = 1.0 / (2 * (p.mob_c * (4 - p.m5) - p.a5inv))
         return kb, kc
+
+
+def extract_cb(canon4d, dims, parity):
+    """orc_fermion_extract_cb: one CANONICAL 4D slice -> checkerboard half."""
+    half = np.zeros(canon4d.size // 2)
+    lib().orc_fermion_extract_cb(half, np.ascontiguousarray(canon4d, np.float64),
+                                 (C.c_int * 4)(*dims), parity)
+    return half
+
+
+def plaquette_linktrace(links, dims):
+    """Lattice::SumReTrPlaq / (18 V) and sum ReTr U / (12 V) of CANONICAL links
+    (no boundary signs), as ReadLatticeParallel::CheckPlaqLinktrace forms them
+    (ReadLatticePar.C:403, 420-428; lattice_base.C:2736-2778, 2878-2905).
+    numpy, FP64."""
+    lx, ly, lz, lt = dims
+    u = np.asarray(links, np.float64).reshape(lt, lz, ly, lx, 4, 3, 3, 2)
+    u = u[..., 0] + 1j * u[..., 1]                 # [t,z,y,x][mu][row][col]
+    axis = {0: 3, 1: 2, 2: 1, 3: 0}                # mu -> array axis of that direction
+    plaq = 0.0
+    for mu in range(4):
+        for nu in range(mu + 1, 4):
+            umu, unu = u[..., mu, :, :], u[..., nu, :, :]
+            unu_xpmu = np.roll(unu, -1, axis=axis[mu])  # U_nu(x + mu)
+            umu_xpnu = np.roll(umu, -1, axis=axis[nu])  # U_mu(x + nu)
+            p = umu @ unu_xpmu @ np.conj(np.swapaxes(umu_xpnu, -1, -2)) @ \
+                np.conj(np.swapaxes(unu, -1, -2))
+            plaq += np.trace(p, axis1=-2, axis2=-1).real.sum()
+    v4 = lx * ly * lz * lt
+    tr = np.trace(u, axis1=-2, axis2=-1).real.sum()
+    return plaq / (18.0 * v4), tr / (12.0 * v4)
