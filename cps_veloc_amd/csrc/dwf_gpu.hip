@@ -658,16 +658,16 @@ struct Ops {
 // |r_s| falls below delta times |r| at the previous update, and whenever the
 // sloppy residual meets the stop condition; the solve ends only when the
 // FP64 true residual meets it (stop rule of inv_cg.C:174,354).
-// The correction x_s is kept in FP32, or in int16 with per-site scales when
-// the sloppy vectors are int16: it only has to be accurate relative to the
-// residual at the last reliable update, and its rounding (1.5e-5 of the site
-// maximum per step) stays far below the factor delta each cycle gains.
+// The correction x_s is kept in FP32 for every sloppy policy: an int16 x_s
+// (per-site scale) rounds each step's increment to 1/32767 of the site's
+// accumulated correction, which on the 24^3x64 Mobius system raised the
+// iteration count from 525 to 661.
 template <class PS>
 int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec& sol,
                 const DVec& src, double* true_res, int* iters, int* nreliable) {
   typedef Ops<P64> H;
   typedef Ops<PS> L;
-  typedef typename std::conditional<std::is_same<PS, PI16>::value, PI16, P32>::type XS;
+  typedef P32 XS;
   TRY(ensure_work<P64>(h, 2));
   TRY(ensure_work<PS>(h, 3));
   TRY(ensure_work<P32>(h, 4));
@@ -675,8 +675,7 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   const DVec& ax = h->f[DWFGPU_F64].work[1];
   const DVec& rs = h->f[pidx<PS>()].work[0];
   const DVec& ps = h->f[pidx<PS>()].work[1];
-  const DVec& xs = std::is_same<XS, PI16>::value ? h->f[pidx<PS>()].work[2]
-                                                  : h->f[DWFGPU_F32].work[3];
+  const DVec& xs = h->f[DWFGPU_F32].work[3];
   const double mass = arg->mass;
   double src_norm_sq;
   TRY(H::norm(h, src, &src_norm_sq));
@@ -1966,7 +1965,354 @@ int minvcg_run(dwfgpu_s* h, double mass, const double* shift, int nshift, const 
   return DWFGPU_OK;
 }
 
+// ---------------------------------------------------------------------------
+// eigCG: DiracOp::InvEigCg (d_op_base/comsrc/inv_eig_cg.C:129-631) on an FP64
+// handle, and the deflation-space bookkeeping of QPropW::eig_Run
+// (alg/alg_qpropw/eig_QPropW.C:85-330).  The vectors (solution, residual,
+// the eigCG window V, the deflation space U) stay in HBM; the small dense
+// problems (the window's tridiagonal T, its projection H, invH) are solved on
+// the host, where they are a few hundred numbers.
+// ---------------------------------------------------------------------------
+// Symmetric eigenproblem of the n x n matrix a (row-major, overwritten) by
+// cyclic Jacobi rotations: w ascending, z[i * n + k] = component i of the
+// k-th eigenvector.
+void sym_eig(std::vector<double> a, int n, std::vector<double>& w, std::vector<double>& z) {
+  z.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) {
+        const double v = a[(size_t)i * n + j] * a[(size_t)i * n + j];
+        tot += v;
+        if (i != j) off += v;
+      }
+    if (off <= 1e-30 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        const double apq = a[(size_t)p * n + q];
+        if (apq == 0.0) continue;
+        const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), sn = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p, q
+          const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+          a[(size_t)k * n + p] = c * akp - sn * akq;
+          a[(size_t)k * n + q] = sn * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p, q
+          const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+          a[(size_t)p * n + k] = c * apk - sn * aqk;
+          a[(size_t)q * n + k] = sn * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          const double zkp = z[(size_t)k * n + p], zkq = z[(size_t)k * n + q];
+          z[(size_t)k * n + p] = c * zkp - sn * zkq;
+          z[(size_t)k * n + q] = sn * zkp + c * zkq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(),
+            [&](int x, int y) { return a[(size_t)x * n + x] < a[(size_t)y * n + y]; });
+  std::vector<double> zs((size_t)n * n);
+  w.resize(n);
+  for (int k = 0; k < n; ++k) {
+    w[k] = a[(size_t)idx[k] * n + idx[k]];
+    for (int i = 0; i < n; ++i) zs[(size_t)i * n + k] = z[(size_t)i * n + idx[k]];
+  }
+  z.swap(zs);
+}
+
+// inverse of the complex n x n matrix a (row-major) by Gauss-Jordan with
+// partial pivoting (the role of invert_H_matrix, eig_QPropW.C:33-83)
+bool cinverse(std::vector<std::complex<double>>& a, int n) {
+  std::vector<std::complex<double>> inv((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) inv[(size_t)i * n + i] = 1.0;
+  for (int c = 0; c < n; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < n; ++r)
+      if (std::abs(a[(size_t)r * n + c]) > std::abs(a[(size_t)piv * n + c])) piv = r;
+    if (std::abs(a[(size_t)piv * n + c]) == 0.0) return false;
+    if (piv != c)
+      for (int k = 0; k < n; ++k) {
+        std::swap(a[(size_t)c * n + k], a[(size_t)piv * n + k]);
+        std::swap(inv[(size_t)c * n + k], inv[(size_t)piv * n + k]);
+      }
+    const std::complex<double> d = 1.0 / a[(size_t)c * n + c];
+    for (int k = 0; k < n; ++k) {
+      a[(size_t)c * n + k] *= d;
+      inv[(size_t)c * n + k] *= d;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const std::complex<double> f = a[(size_t)r * n + c];
+      if (f == 0.0) continue;
+      for (int k = 0; k < n; ++k) {
+        a[(size_t)r * n + k] -= f * a[(size_t)c * n + k];
+        inv[(size_t)r * n + k] -= f * inv[(size_t)c * n + k];
+      }
+    }
+  }
+  a.swap(inv);
+  return true;
+}
+
+struct EigWork {
+  double* buf = nullptr;   // device: qz / coefficients
+  double* red = nullptr;   // device: reduced dot products
+  double* part = nullptr;  // device: per-block partial dot products
+  ~EigWork() {
+    if (buf) (void)hipFree(buf);
+    if (red) (void)hipFree(red);
+    if (part) (void)hipFree(part);
+  }
+  int init(int m) {
+    const size_t nbuf = std::max((size_t)m * dwfk::EIG_MAXRANK, (size_t)2 * dwfk::EIG_DOTK);
+    HIPCHK(hipMalloc(&buf, nbuf * sizeof(double)));
+    HIPCHK(hipMalloc(&red, 2 * dwfk::EIG_DOTK * sizeof(double)));
+    HIPCHK(hipMalloc(&part, (size_t)2 * dwfk::EIG_DOTK * dwfk::EIG_BLOCKS * sizeof(double)));
+    return DWFGPU_OK;
+  }
+};
+
+// out[k] = <W_k, x> (complex) for k < K; W: K vectors `stride` reals apart
+int eig_mdot(dwfgpu_s* h, EigWork& w, const double* W, size_t stride, int K, const double* x,
+             std::vector<std::complex<double>>& out) {
+  out.assign(K, 0.0);
+  const int nb = dwfk::EIG_BLOCKS;
+  std::vector<double> hv(2 * dwfk::EIG_DOTK);
+  for (int k0 = 0; k0 < K; k0 += dwfk::EIG_DOTK) {
+    const int kk = std::min(dwfk::EIG_DOTK, K - k0);
+    hipLaunchKernelGGL(dwfk::k_eig_mdot, dim3(nb), dim3(256), 0, h->st, W + (size_t)k0 * stride,
+                       stride, kk, x, h->n / 2, w.part);
+    hipLaunchKernelGGL(dwfk::k_eig_reduce_rows, dim3(2 * kk), dim3(256), 0, h->st, w.part, nb,
+                       w.red);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(hv.data(), w.red, 2 * kk * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    for (int k = 0; k < kk; ++k) out[k0 + k] = {hv[2 * k], hv[2 * k + 1]};
+  }
+  return DWFGPU_OK;
+}
+
+// out += sum_k c_k W_k
+int eig_mcombo(dwfgpu_s* h, EigWork& w, double* out, const double* W, size_t stride, int K,
+               const std::vector<std::complex<double>>& c) {
+  for (int k0 = 0; k0 < K; k0 += dwfk::EIG_DOTK) {
+    const int kk = std::min(dwfk::EIG_DOTK, K - k0);
+    std::vector<double> hv(2 * kk);
+    for (int k = 0; k < kk; ++k) {
+      hv[2 * k] = c[k0 + k].real();
+      hv[2 * k + 1] = c[k0 + k].imag();
+    }
+    HIPCHK(hipMemcpyAsync(w.buf, hv.data(), 2 * kk * sizeof(double), hipMemcpyHostToDevice, h->st));
+    hipLaunchKernelGGL(dwfk::k_eig_mcombo, blas_grid(), blas_block(), 0, h->st, out,
+                       W + (size_t)k0 * stride, stride, kk, w.buf, h->n / 2);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));  // w.buf is reused by the next chunk
+  }
+  return DWFGPU_OK;
+}
+
+// e = U invH U^dag v  (the init-CG guess of inv_eig_cg.C:202-225, 514-537)
+int eig_deflate(dwfgpu_s* h, EigWork& w, const DVec& e, const double* U, int def_len,
+                const std::vector<std::complex<double>>& invH, const DVec& v) {
+  std::vector<std::complex<double>> ub, c(def_len, 0.0);
+  TRY(eig_mdot(h, w, U, h->n, def_len, (const double*)v.d, ub));
+  for (int i = 0; i < def_len; ++i)
+    for (int j = 0; j < def_len; ++j) c[i] += invH[(size_t)i * def_len + j] * ub[j];
+  HIPCHK(hipMemsetAsync(e.d, 0, h->n * sizeof(double), h->st));
+  return eig_mcombo(h, w, (double*)e.d, U, h->n, def_len, c);
+}
+
+int inveigcg_run(dwfgpu_s* h, const dwfgpu_cg_arg* arg, const DVec& sol, const DVec& src,
+                 int nev, int m, double* V, double* M, const double* U,
+                 const std::vector<std::complex<double>>& invH, int def_len,
+                 const double* restart, int restart_len, double* true_res, int* iters) {
+  typedef Ops<P64> H;
+  TRY(ensure_work<P64>(h, 6));
+  const DVec& r = h->f[DWFGPU_F64].work[3];
+  const DVec& dir = h->f[DWFGPU_F64].work[4];
+  const DVec& mmp = h->f[DWFGPU_F64].work[5];
+  DVec mmp_prev;
+  struct Free {
+    DVec* v;
+    ~Free() { free_vec(v); }
+  } fr{&mmp_prev};
+  if (nev > 0) TRY(alloc_vec<P64>(h, &mmp_prev));
+  EigWork w;
+  TRY(w.init(m));
+  const size_t n = h->n;
+  const double mass = arg->mass;
+  double src_norm_sq;
+  TRY(H::norm(h, src, &src_norm_sq));
+  const double stp_cnd = src_norm_sq * arg->stop_rsd * arg->stop_rsd;
+  // initial guess: 0, or U invH U^dag src
+  if (def_len > 0)
+    TRY(eig_deflate(h, w, sol, U, def_len, invH, src));
+  else
+    HIPCHK(hipMemsetAsync(sol.d, 0, n * sizeof(double), h->st));
+  TRY(H::mdagm(h, mass, mmp, sol, nullptr));
+  TRY(H::axpy_to(h, r, mmp, src, -1.0, true));  // r = src - A sol
+  double rr;
+  TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
+  TRY(H::copy(h, dir, r));
+  // eigCG state: T (m x m, symmetric, full storage here), the window count
+  const int nev2 = 2 * nev;
+  std::vector<double> T((size_t)m * m, 0.0);
+  if (nev > 0)
+    for (int i = 0; i < nev2; ++i) M[i] = 0.0;
+  int i_eig = 0;
+  double alpha = 1.0, beta = 0.0, alpha_old = 0.0, beta_old = 0.0;
+  auto Tij = [&](int i, int j) -> double& { return T[(size_t)i * m + j]; };
+  auto vptr = [&](int k) { return V + (size_t)k * n; };
+  std::vector<double> restartcond(restart_len);
+  for (int i = 0; i < restart_len; ++i) restartcond[i] = restart[i] * restart[i] * src_norm_sq;
+  int restarted = 0;
+  int iter;
+  bool breakdown = false;
+  const int max_iter = arg->max_num_iter;
+  for (iter = 0; iter < max_iter; iter++) {
+    if (nev > 0 && i_eig == m) TRY(H::copy(h, mmp_prev, mmp));
+    const double rr_prv = rr;
+    double pap;
+    TRY(H::mdagm(h, mass, mmp, dir, &pap));
+    if (pap == 0.0) {
+      breakdown = true;
+      break;
+    }
+    if (nev > 0) {
+      if (i_eig != 0) Tij(i_eig - 1, i_eig - 1) = 1.0 / alpha + beta_old / alpha_old;
+      if (i_eig == m) {
+        // the window is full: keep the nev lowest Ritz vectors of T(m) and of
+        // T(m-1) (inv_eig_cg.C:318-411), orthonormalised, and the Ritz pairs
+        // of T in their span
+        std::vector<double> Q((size_t)m * nev2, 0.0), ev, Y;
+        std::vector<double> Tm1((size_t)(m - 1) * (m - 1));
+        for (int i = 0; i < m - 1; ++i)
+          for (int j = 0; j < m - 1; ++j) Tm1[(size_t)i * (m - 1) + j] = Tij(i, j);
+        sym_eig(Tm1, m - 1, ev, Y);
+        for (int c = 0; c < nev; ++c)
+          for (int j = 0; j < m - 1; ++j) Q[(size_t)j * nev2 + nev + c] = Y[(size_t)j * (m - 1) + c];
+        sym_eig(T, m, ev, Y);
+        for (int c = 0; c < nev; ++c)
+          for (int j = 0; j < m; ++j) Q[(size_t)j * nev2 + c] = Y[(size_t)j * m + c];
+        int rank = nev;
+        for (int i = nev; i < nev2; ++i) {
+          for (int j = 0; j < rank; ++j) {
+            double xy = 0.0;
+            for (int k = 0; k < m; ++k) xy += Q[(size_t)k * nev2 + i] * Q[(size_t)k * nev2 + j];
+            for (int k = 0; k < m; ++k) Q[(size_t)k * nev2 + i] -= xy * Q[(size_t)k * nev2 + j];
+          }
+          double xx = 0.0;
+          for (int k = 0; k < m; ++k) xx += Q[(size_t)k * nev2 + i] * Q[(size_t)k * nev2 + i];
+          if (xx > 1e-16) {
+            xx = 1.0 / std::sqrt(xx);
+            for (int k = 0; k < m; ++k) Q[(size_t)k * nev2 + rank] = xx * Q[(size_t)k * nev2 + i];
+            rank++;
+          }
+        }
+        // Hq = Q^T T Q (rank x rank), its eigenpairs, QZ = Q Z (m x rank)
+        std::vector<double> TQ((size_t)m * rank, 0.0), Hq((size_t)rank * rank, 0.0);
+        for (int l = 0; l < m; ++l)
+          for (int j = 0; j < rank; ++j) {
+            double a = 0.0;
+            for (int k = 0; k < m; ++k) a += Tij(l, k) * Q[(size_t)k * nev2 + j];
+            TQ[(size_t)l * rank + j] = a;
+          }
+        for (int i = 0; i < rank; ++i)
+          for (int j = 0; j < rank; ++j) {
+            double a = 0.0;
+            for (int l = 0; l < m; ++l) a += Q[(size_t)l * nev2 + i] * TQ[(size_t)l * rank + j];
+            Hq[(size_t)i * rank + j] = a;
+          }
+        std::vector<double> mz, Z;
+        sym_eig(Hq, rank, mz, Z);
+        for (int i = 0; i < nev2; ++i) M[i] = i < rank ? mz[i] : 0.0;
+        std::vector<double> qz((size_t)m * rank, 0.0);  // qz[i + m j]
+        for (int j = 0; j < rank; ++j)
+          for (int i = 0; i < m; ++i) {
+            double a = 0.0;
+            for (int k = 0; k < rank; ++k) a += Q[(size_t)i * nev2 + k] * Z[(size_t)k * rank + j];
+            qz[i + (size_t)m * j] = a;
+          }
+        HIPCHK(hipMemcpyAsync(w.buf, qz.data(), qz.size() * sizeof(double), hipMemcpyHostToDevice,
+                              h->st));
+        hipLaunchKernelGGL(dwfk::k_eig_vmul, blas_grid(), blas_block(),
+                           (size_t)m * rank * sizeof(double), h->st, V, n, n, m, rank, w.buf);
+        HIPCHK(hipGetLastError());
+        std::fill(T.begin(), T.end(), 0.0);
+        for (int i = 0; i < rank; ++i) Tij(i, i) = M[i];
+        i_eig = rank;
+        // w = A r = mmp - beta mmp_prev; T(rank, ii) = <w, V_ii> / |r|
+        TRY(H::axpby(h, mmp_prev, mmp, 1.0, -beta));
+        std::vector<std::complex<double>> tw;
+        TRY(eig_mdot(h, w, V, n, rank, (const double*)mmp_prev.d, tw));
+        for (int ii = 0; ii < i_eig; ++ii) {
+          Tij(i_eig, ii) = tw[ii].real() / std::sqrt(rr);
+          Tij(ii, i_eig) = Tij(i_eig, ii);
+        }
+      } else if (i_eig != 0) {
+        Tij(i_eig, i_eig - 1) = -std::sqrt(beta) / alpha;
+        Tij(i_eig - 1, i_eig) = Tij(i_eig, i_eig - 1);
+      }
+      // V[i_eig] = r / |r|
+      DVec vi;
+      vi.d = vptr(i_eig);
+      TRY(H::copy(h, vi, r));
+      TRY(H::scale(h, vi, 1.0 / std::sqrt(rr)));
+      i_eig++;
+    }
+    alpha_old = alpha;
+    alpha = -rr_prv / pap;
+    TRY(H::axpy_to(h, r, mmp, r, alpha, true));  // r += alpha A p, |r|^2
+    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
+    alpha = -alpha;
+    beta_old = beta;
+    beta = rr / rr_prv;
+    // sol += alpha dir ; dir = r + beta dir
+    hipLaunchKernelGGL((dwfk::k_cg_xp<P64, P64, true>), blas_grid(), blas_block(), 0, h->st,
+                       fld<P64>(sol), fld<P64>(dir), cfld<P64>(r), alpha, beta, h->g);
+    HIPCHK(hipGetLastError());
+    if (restarted < restart_len && def_len > 0 && rr < restartcond[restarted]) {
+      // init-CG again on the current residual (inv_eig_cg.C:504-559)
+      restarted++;
+      TRY(eig_deflate(h, w, dir, U, def_len, invH, r));
+      TRY(H::axpby(h, sol, dir, 1.0, 1.0));
+      TRY(H::mdagm(h, mass, mmp, dir, nullptr));
+      TRY(H::axpy_to(h, r, mmp, r, -1.0, true));
+      TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
+      TRY(H::copy(h, dir, r));
+      TRY(H::copy(h, mmp, r));  // as CPS, whose mmp holds the residual here
+    }
+    if (rr <= stp_cnd) break;
+  }
+  // true residual (inv_eig_cg.C:590-601)
+  TRY(H::mdagm(h, mass, mmp, sol, nullptr));
+  TRY(H::axpy_to(h, r, mmp, src, -1.0, true));
+  double tr;
+  TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &tr));
+  if (true_res) *true_res = std::sqrt(tr / src_norm_sq);
+  if (iters) *iters = iter;
+  if (breakdown) return DWFGPU_BREAKDOWN;
+  return iter == max_iter ? DWFGPU_MAXITER : DWFGPU_OK;
+}
+
 }  // namespace
+
+// the deflation space of QPropW::eig_Run: U (float-valued FP64 vectors on the
+// device), H = U^dag A U, the eigCG window V and its Ritz values M
+struct dwfgpu_eigcg_s {
+  dwfgpu_s* h = nullptr;
+  int nev = 0, m = 0, max_def_len = 0, def_len = 0;
+  double max_eig = 0.0;
+  DVec U, V;
+  std::vector<std::complex<double>> H;  // max_def_len^2, row-major
+  std::vector<double> M;
+};
 
 // ===========================================================================
 // C ABI
@@ -2800,6 +3146,156 @@ int dwfgpu_plaquette(dwfgpu_t h, double* plaq, double* link_trace) {
   // Lattice::SumReTrPlaq / (18 V), sum ReTr U / (12 V) (ReadLatticePar.C:403,428)
   if (plaq) *plaq = sc[0] / (18.0 * v4);
   if (link_trace) *link_trace = sc[1] / (12.0 * v4);
+  return DWFGPU_OK;
+}
+
+// ---- (f4) eigCG deflation (inv_eig_cg.C:129-631, eig_QPropW.C:85-330) -----
+int dwfgpu_inveigcg_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, dwfgpu_vec_t sol, dwfgpu_vec_t src,
+                        int nev, int m, dwfgpu_mvec_t V, double* M, dwfgpu_mvec_t U,
+                        const double* invH, int def_len, const double* restart, int restart_len,
+                        double* true_res, int* iters) {
+  if (!own_pair(h, sol, src) || !arg || arg->max_num_iter < 1 || nev < 0 || def_len < 0 ||
+      restart_len < 0 || (restart_len > 0 && !restart)) {
+    set_err("dwfgpu_inveigcg_dev: bad argument");
+    return DWFGPU_BAD_ARG;
+  }
+  if (h->prec != DWFGPU_F64) {
+    set_err("dwfgpu_inveigcg_dev: needs an FP64 handle");
+    return DWFGPU_BAD_ARG;
+  }
+  if (nev > 0 && (m <= 2 * nev || m > 256 || 2 * nev > dwfk::EIG_MAXRANK || !V || V->h != h ||
+                  V->nrhs < m || !M)) {
+    set_err("dwfgpu_inveigcg_dev: 2 nev < m <= 256, 2 nev <= %d and an mvec of m vectors needed",
+            dwfk::EIG_MAXRANK);
+    return DWFGPU_BAD_ARG;
+  }
+  if (def_len > 0 && (!U || U->h != h || U->nrhs < def_len || !invH)) {
+    set_err("dwfgpu_inveigcg_dev: U with def_len vectors and invH needed");
+    return DWFGPU_BAD_ARG;
+  }
+  std::vector<std::complex<double>> ih((size_t)def_len * def_len);
+  for (size_t k = 0; k < ih.size(); ++k) ih[k] = {invH[2 * k], invH[2 * k + 1]};
+  return inveigcg_run(h, arg, sol->v, src->v, nev, nev > 0 ? m : 0,
+                      nev > 0 ? (double*)V->v.d : nullptr, M,
+                      def_len > 0 ? (const double*)U->v.d : nullptr, ih, def_len, restart,
+                      restart_len, true_res, iters);
+}
+
+int dwfgpu_eigcg_create(dwfgpu_t h, int nev, int m, int max_def_len, double max_eig,
+                        dwfgpu_eigcg_t* out) {
+  if (!h || !out || nev < 1 || m <= 2 * nev || m > 256 || 2 * nev > dwfk::EIG_MAXRANK ||
+      max_def_len < 1) {
+    set_err("dwfgpu_eigcg_create: nev >= 1, 2 nev < m <= 256, 2 nev <= %d, max_def_len >= 1",
+            dwfk::EIG_MAXRANK);
+    return DWFGPU_BAD_ARG;
+  }
+  if (h->prec != DWFGPU_F64) {
+    set_err("dwfgpu_eigcg_create: needs an FP64 handle");
+    return DWFGPU_BAD_ARG;
+  }
+  dwfgpu_eigcg_s* e = new dwfgpu_eigcg_s;
+  e->h = h;
+  e->nev = nev;
+  e->m = m;
+  e->max_def_len = max_def_len;
+  e->max_eig = max_eig;
+  e->H.assign((size_t)max_def_len * max_def_len, 0.0);
+  e->M.assign(2 * nev, 0.0);
+  int st = alloc_mvec<P64>(h, m, &e->V);
+  if (!st) st = alloc_mvec<P64>(h, max_def_len, &e->U);
+  if (st) {
+    free_vec(&e->V);
+    free_vec(&e->U);
+    delete e;
+    return st;
+  }
+  *out = e;
+  return DWFGPU_OK;
+}
+
+int dwfgpu_eigcg_destroy(dwfgpu_eigcg_t e) {
+  if (!e) return DWFGPU_BAD_ARG;
+  free_vec(&e->V);
+  free_vec(&e->U);
+  delete e;
+  return DWFGPU_OK;
+}
+
+int dwfgpu_eigcg_def_len(dwfgpu_eigcg_t e) { return e ? e->def_len : -1; }
+
+int dwfgpu_eigcg_solve(dwfgpu_eigcg_t e, const dwfgpu_cg_arg* arg, dwfgpu_vec_t sol,
+                       dwfgpu_vec_t src, const double* restart, int restart_len,
+                       int always_restart, double* true_res, int* iters) {
+  if (!e || !arg || !own_pair(e->h, sol, src) || restart_len < 0 ||
+      (restart_len > 0 && !restart) || arg->max_num_iter < 1)
+    return DWFGPU_BAD_ARG;
+  dwfgpu_s* h = e->h;
+  const size_t n = h->n;
+  const int dl = e->def_len;
+  // invH from the leading def_len x def_len block of H (eig_QPropW.C:253-282)
+  std::vector<std::complex<double>> ih((size_t)dl * dl);
+  for (int i = 0; i < dl; ++i)
+    for (int j = 0; j < dl; ++j) ih[(size_t)i * dl + j] = e->H[(size_t)i * e->max_def_len + j];
+  if (dl > 0 && !cinverse(ih, dl)) {
+    set_err("dwfgpu_eigcg_solve: U^dag A U is singular");
+    return DWFGPU_BAD_ARG;
+  }
+  // window and restarts as eig_Run chooses them (eig_QPropW.C:283-290)
+  int nev = e->nev, rl = restart_len;
+  if (!always_restart && dl < e->max_def_len)
+    rl = 0;
+  else if (dl == e->max_def_len)
+    nev = 0;
+  int st = inveigcg_run(h, arg, sol->v, src->v, nev, nev > 0 ? e->m : 0, (double*)e->V.d,
+                        e->M.data(), (const double*)e->U.d, ih, dl, restart, rl, true_res, iters);
+  if (st != DWFGPU_OK && st != DWFGPU_MAXITER) return st;
+  if (nev == 0 || e->def_len >= e->max_def_len) return st;
+  // add the new low Ritz vectors, lowest first, to U and H (eig_QPropW.C:292-330)
+  const int nev2 = 2 * e->nev;
+  std::vector<int> low(nev2);
+  for (int i = 0; i < nev2; ++i) low[i] = i;
+  std::stable_sort(low.begin(), low.end(), [&](int a, int b) { return e->M[a] < e->M[b]; });
+  const DVec& au = h->f[DWFGPU_F64].work[5];
+  EigWork w;
+  TRY(w.init(1));
+  for (int i = 0; i < nev2; ++i) {
+    const int k = low[i];
+    if (!(e->M[k] < e->max_eig && e->M[k] > 1e-30 && e->def_len < e->max_def_len)) continue;
+    const int d = e->def_len;
+    DVec vk, ud;
+    vk.d = (double*)e->V.d + (size_t)k * n;
+    ud.d = (double*)e->U.d + (size_t)d * n;
+    hipLaunchKernelGGL(dwfk::k_eig_round_f32, blas_grid(), blas_block(), 0, h->st,
+                       (double*)ud.d, (double*)vk.d, n);
+    HIPCHK(hipGetLastError());
+    TRY(Ops<P64>::mdagm(h, arg->mass, au, vk, nullptr));
+    if (d > 0) {
+      std::vector<std::complex<double>> c;
+      TRY(eig_mdot(h, w, (const double*)e->U.d, n, d, (const double*)au.d, c));
+      for (int j = 0; j < d; ++j) {
+        e->H[(size_t)j * e->max_def_len + d] = c[j];
+        e->H[(size_t)d * e->max_def_len + j] = std::conj(c[j]);
+      }
+    }
+    double hd;
+    TRY(dot<P64>(h, vk, au, &hd));
+    e->H[(size_t)d * e->max_def_len + d] = hd;
+    e->def_len++;
+  }
+  HIPCHK(hipStreamSynchronize(h->st));
+  return st;
+}
+
+int dwfgpu_eigcg_values(dwfgpu_eigcg_t e, double* M, double* H) {
+  if (!e) return DWFGPU_BAD_ARG;
+  if (M) std::memcpy(M, e->M.data(), e->M.size() * sizeof(double));
+  if (H)
+    for (int i = 0; i < e->def_len; ++i)
+      for (int j = 0; j < e->def_len; ++j) {
+        const std::complex<double> v = e->H[(size_t)i * e->max_def_len + j];
+        H[2 * ((size_t)i * e->def_len + j)] = v.real();
+        H[2 * ((size_t)i * e->def_len + j) + 1] = v.imag();
+      }
   return DWFGPU_OK;
 }
 

@@ -1651,4 +1651,113 @@ k_xpby_mrhs(Fld<P> p, CFld<P> r, const double* __restrict__ scal, const int* __r
   }
 }
 
+// ---------------------------------------------------------------------------
+// eigCG (DiracOp::InvEigCg, d_op_base/comsrc/inv_eig_cg.C:129-631) and the
+// init-CG deflation.  These kernels treat FP64 vectors as flat arrays of n
+// reals (the device layout permutes sites and components identically in every
+// vector, so sums and linear combinations do not see it).  A set of K vectors
+// is a base pointer with `stride` reals between vectors.
+// ---------------------------------------------------------------------------
+constexpr int EIG_MAXRANK = 32;  // 2 * nev
+constexpr int EIG_DOTK = 16;     // vectors per batched-dot launch
+constexpr int EIG_BLOCKS = 512;
+
+// V[j] = sum_k V[k] qz[k + m j] for j < rank, in place (each thread reads
+// every V[k] at its own index before it writes any V[j] there)
+__global__ void __launch_bounds__(256)
+k_eig_vmul(double* __restrict__ V, size_t stride, size_t n, int m, int rank,
+           const double* __restrict__ qz) {
+  extern __shared__ double sqz[];  // [m * rank]
+  for (int t = threadIdx.x; t < m * rank; t += blockDim.x) sqz[t] = qz[t];
+  __syncthreads();
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    double acc[EIG_MAXRANK];
+#pragma unroll
+    for (int j = 0; j < EIG_MAXRANK; ++j) acc[j] = 0.0;
+    for (int k = 0; k < m; ++k) {
+      const double v = V[k * stride + e];
+#pragma unroll
+      for (int j = 0; j < EIG_MAXRANK; ++j)
+        if (j < rank) acc[j] = fma(v, sqz[k + m * j], acc[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < EIG_MAXRANK; ++j)
+      if (j < rank) V[j * stride + e] = acc[j];
+  }
+}
+
+// per block: sum over its complex elements of conj(W_k) x, k < K <= EIG_DOTK;
+// partial[(2k + c) * gridDim.x + block], c = 0 real, 1 imaginary
+__global__ void __launch_bounds__(256)
+k_eig_mdot(const double* __restrict__ W, size_t stride, int K, const double* __restrict__ x,
+           size_t ncx, double* __restrict__ partial) {
+  __shared__ double sh[16];
+  double re[EIG_DOTK], im[EIG_DOTK];
+#pragma unroll
+  for (int k = 0; k < EIG_DOTK; ++k) re[k] = im[k] = 0.0;
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < ncx;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const double xr = x[2 * e], xi = x[2 * e + 1];
+#pragma unroll
+    for (int k = 0; k < EIG_DOTK; ++k) {
+      if (k < K) {
+        const double wr = W[k * stride + 2 * e], wi = W[k * stride + 2 * e + 1];
+        re[k] = fma(wr, xr, fma(wi, xi, re[k]));
+        im[k] = fma(wr, xi, fma(-wi, xr, im[k]));
+      }
+    }
+  }
+  for (int k = 0; k < K; ++k) {
+    const double a = block_sum(re[k], sh);
+    __syncthreads();
+    const double b = block_sum(im[k], sh);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      partial[(2 * k) * gridDim.x + blockIdx.x] = a;
+      partial[(2 * k + 1) * gridDim.x + blockIdx.x] = b;
+    }
+  }
+}
+
+// out[v] = sum of partial[v * nb .. v * nb + nb), one block per value
+__global__ void __launch_bounds__(256)
+k_eig_reduce_rows(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+  __shared__ double sh[16];
+  double a = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) a += partial[(size_t)blockIdx.x * nb + b];
+  const double t = block_sum(a, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// out += sum_k c_k W_k with complex c_k (coef[2k], coef[2k+1]), k < K
+__global__ void __launch_bounds__(256)
+k_eig_mcombo(double* __restrict__ out, const double* __restrict__ W, size_t stride, int K,
+             const double* __restrict__ coef, size_t ncx) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < ncx;
+       e += (size_t)gridDim.x * blockDim.x) {
+    double orr = out[2 * e], oi = out[2 * e + 1];
+    for (int k = 0; k < K; ++k) {
+      const double cr = coef[2 * k], ci = coef[2 * k + 1];
+      const double wr = W[k * stride + 2 * e], wi = W[k * stride + 2 * e + 1];
+      orr = fma(cr, wr, fma(-ci, wi, orr));
+      oi = fma(cr, wi, fma(ci, wr, oi));
+    }
+    out[2 * e] = orr;
+    out[2 * e + 1] = oi;
+  }
+}
+
+// u = v = (double)(float)v: the deflation space is kept at single precision
+// (eig_QPropW.C:315)
+__global__ void __launch_bounds__(256)
+k_eig_round_f32(double* __restrict__ u, double* __restrict__ v, size_t n) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const double r = (double)(float)v[e];
+    u[e] = r;
+    v[e] = r;
+  }
+}
+
 }  // namespace dwfk

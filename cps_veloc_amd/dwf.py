@@ -47,6 +47,8 @@ EXPORTS = (
     "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
     "dwfgpu_fermion_from_canonical", "dwfgpu_fermion_to_canonical", "dwfgpu_four2five",
     "dwfgpu_five2four", "dwfgpu_matinv_dev", "dwfgpu_nersc_read", "dwfgpu_plaquette",
+    "dwfgpu_inveigcg_dev", "dwfgpu_eigcg_create", "dwfgpu_eigcg_solve", "dwfgpu_eigcg_def_len",
+    "dwfgpu_eigcg_values", "dwfgpu_eigcg_destroy",
 )
 # every entry point of include/dwf_synth.h
 SYNTH_EXPORTS = ("dwfsynth_gauge", "dwfsynth_gaussian")
@@ -167,6 +169,16 @@ def lib():
         L.dwfgpu_nersc_read.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t,
                                         C.POINTER(NerscHeaderC)]
         L.dwfgpu_plaquette.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.dwfgpu_inveigcg_dev.argtypes = [H, C.POINTER(CgArgC), H, H, C.c_int, C.c_int, H,
+                                          C.c_void_p, H, C.c_void_p, C.c_int, C.c_void_p,
+                                          C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.dwfgpu_eigcg_create.argtypes = [H, C.c_int, C.c_int, C.c_int, C.c_double,
+                                          C.POINTER(H)]
+        L.dwfgpu_eigcg_solve.argtypes = [H, C.POINTER(CgArgC), H, H, C.c_void_p, C.c_int,
+                                         C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.dwfgpu_eigcg_def_len.argtypes = [H]
+        L.dwfgpu_eigcg_values.argtypes = [H, C.c_void_p, C.c_void_p]
+        L.dwfgpu_eigcg_destroy.argtypes = [H]
         L.dwfsynth_gauge.argtypes = [_dp, C.c_int * 4, C.c_double, C.c_ulonglong]
         L.dwfsynth_gauge.restype = None
         L.dwfsynth_gaussian.argtypes = [_dp, C.c_size_t, C.c_ulonglong]
@@ -510,11 +522,79 @@ class _DiracOpGpu:
                   "MatInv_dev", allow=(OK, MAXITER))
         return it.value, tr.value, st
 
+    # --- eigCG (inv_eig_cg.C:129-631) -------------------------------------
+    def InvEigCg_dev(self, sol: DeviceVector, src: DeviceVector, cg_arg: CgArg, nev=0, m=0,
+                     V: MultiVector = None, U: MultiVector = None, invH=None, restart=()):
+        """DiracOp::InvEigCg.  Returns (iters, true_res, status, M)."""
+        tr, it = C.c_double(), C.c_int()
+        a = cg_arg.c()
+        M = np.zeros(max(2 * nev, 1))
+        dl = 0 if invH is None else int(np.asarray(invH).shape[0])
+        ih = None
+        if dl:
+            ih = np.ascontiguousarray(np.asarray(invH, np.complex128)).view(np.float64)
+        rs = np.ascontiguousarray(restart, np.float64) if len(restart) else None
+        st = _chk(lib().dwfgpu_inveigcg_dev(
+            self.h, C.byref(a), sol.h, src.h, nev, m, V.h if V is not None else None,
+            M.ctypes.data_as(C.c_void_p), U.h if U is not None else None,
+            None if ih is None else ih.ctypes.data_as(C.c_void_p), dl,
+            None if rs is None else rs.ctypes.data_as(C.c_void_p), 0 if rs is None else rs.size,
+            C.byref(tr), C.byref(it)), "InvEigCg", allow=(OK, MAXITER))
+        return it.value, tr.value, st, M[:2 * nev]
+
     def plaquette(self):
         """(SumReTrPlaq / (18 V), sum ReTr U / (12 V)) of the handle's links."""
         p, t = C.c_double(), C.c_double()
         _chk(lib().dwfgpu_plaquette(self.h, C.byref(p), C.byref(t)), "plaquette")
         return p.value, t.value
+
+
+class EigCG:
+    """The deflation space of QPropW::eig_Run (eig_QPropW.C:85-330) around
+    InvEigCg: one solve() per right-hand side grows U (up to max_def_len
+    single-precision Ritz vectors) and H = U^dag A U on the device."""
+
+    def __init__(self, op: "_DiracOpGpu", nev, m, max_def_len, max_eig):
+        self.op, self.nev = op, nev
+        self.h = C.c_void_p()
+        _chk(lib().dwfgpu_eigcg_create(op.h, nev, m, max_def_len, max_eig, C.byref(self.h)),
+             "eigcg_create")
+
+    def solve(self, sol: DeviceVector, src: DeviceVector, cg_arg: CgArg, restart=(),
+              always_restart=False):
+        tr, it = C.c_double(), C.c_int()
+        a = cg_arg.c()
+        rs = np.ascontiguousarray(restart, np.float64) if len(restart) else None
+        st = _chk(lib().dwfgpu_eigcg_solve(
+            self.h, C.byref(a), sol.h, src.h,
+            None if rs is None else rs.ctypes.data_as(C.c_void_p), 0 if rs is None else rs.size,
+            1 if always_restart else 0, C.byref(tr), C.byref(it)), "eigcg_solve",
+            allow=(OK, MAXITER))
+        return it.value, tr.value, st
+
+    @property
+    def def_len(self):
+        return int(lib().dwfgpu_eigcg_def_len(self.h))
+
+    def values(self):
+        """(M: the last window's 2 nev Ritz values, H: def_len x def_len complex)."""
+        M = np.zeros(2 * self.nev)
+        dl = self.def_len
+        Hm = np.zeros(max(2 * dl * dl, 1))
+        _chk(lib().dwfgpu_eigcg_values(self.h, M.ctypes.data_as(C.c_void_p),
+                                       Hm.ctypes.data_as(C.c_void_p)), "eigcg_values")
+        return M, Hm[:2 * dl * dl].view(np.complex128).reshape(dl, dl)
+
+    def free(self):
+        if self.h:
+            lib().dwfgpu_eigcg_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 @dataclass

@@ -308,6 +308,40 @@ int dwfgpu_nersc_read(const char *path, double *gauge, size_t n_reals,
  * Either pointer may be NULL.                                                */
 int dwfgpu_plaquette(dwfgpu_t h, double *plaq, double *link_trace);
 
+/* ---- (f4) eigCG deflation ---------------------------------------------- */
+/* DiracOp::InvEigCg (src/util/dirac_op/d_op_base/comsrc/inv_eig_cg.C:129-631),
+ * FP64 handle.  CG on MatPcDagMatPc from sol = U invH U^dag src (def_len > 0;
+ * else 0).  With nev > 0 an eigCG window of m > 2 nev normalised residuals is
+ * kept in V (>= m vectors); each time it fills, the Ritz pairs of the window's
+ * Lanczos matrix in the span of its nev lowest eigenvectors and those of its
+ * leading (m-1) block replace it: V[0..rank) and M[0..rank), M[rank..2nev) = 0
+ * (2 nev <= 32, m <= 256; M is zeroed on entry).  U: >= def_len vectors; invH: def_len x
+ * def_len complex, row-major, re/im interleaved.  restart[0..restart_len):
+ * relative residuals at which the init-CG correction U invH U^dag r is
+ * applied again.  *iters = CPS's iter (loop count); DWFGPU_MAXITER, or
+ * DWFGPU_BREAKDOWN when <p, A p> = 0 (CPS breaks out of the loop).          */
+int dwfgpu_inveigcg_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol, dwfgpu_vec_t src,
+                        int nev, int m, dwfgpu_mvec_t V, double *M, dwfgpu_mvec_t U,
+                        const double *invH, int def_len, const double *restart, int restart_len,
+                        double *true_res, int *iters);
+/* The deflation-space bookkeeping of QPropW::eig_Run (src/alg/alg_qpropw/
+ * eig_QPropW.C:85-330) around InvEigCg, one call per right-hand side:
+ * invH = (U^dag A U)^-1 of the current space, the solve (an eigCG window
+ * while the space is not full; restarts at restart[] when always_restart or
+ * once it is full), then the new Ritz vectors with value in (1e-30, max_eig),
+ * lowest first, rounded to single precision, appended to U with their row
+ * and column of H = U^dag A U, up to max_def_len vectors.                   */
+typedef struct dwfgpu_eigcg_s *dwfgpu_eigcg_t;
+int dwfgpu_eigcg_create(dwfgpu_t h, int nev, int m, int max_def_len, double max_eig,
+                        dwfgpu_eigcg_t *out);
+int dwfgpu_eigcg_solve(dwfgpu_eigcg_t e, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol,
+                       dwfgpu_vec_t src, const double *restart, int restart_len,
+                       int always_restart, double *true_res, int *iters);
+int dwfgpu_eigcg_def_len(dwfgpu_eigcg_t e);
+/* M: the 2 nev Ritz values of the last window; H: def_len^2 complex (re/im). */
+int dwfgpu_eigcg_values(dwfgpu_eigcg_t e, double *M, double *H);
+int dwfgpu_eigcg_destroy(dwfgpu_eigcg_t e);
+
 #ifdef __cplusplus
 }
 #endif
