@@ -1689,8 +1689,9 @@ k_eig_vmul(double* __restrict__ V, size_t stride, size_t n, int m, int rank,
 
 // per block: sum over its complex elements of conj(W_k) x, k < K <= EIG_DOTK;
 // partial[(2k + c) * gridDim.x + block], c = 0 real, 1 imaginary
+template <typename TW>
 __global__ void __launch_bounds__(256)
-k_eig_mdot(const double* __restrict__ W, size_t stride, int K, const double* __restrict__ x,
+k_eig_mdot(const TW* __restrict__ W, size_t stride, int K, const double* __restrict__ x,
            size_t ncx, double* __restrict__ partial) {
   __shared__ double sh[16];
   double re[EIG_DOTK], im[EIG_DOTK];
@@ -1702,7 +1703,7 @@ k_eig_mdot(const double* __restrict__ W, size_t stride, int K, const double* __r
 #pragma unroll
     for (int k = 0; k < EIG_DOTK; ++k) {
       if (k < K) {
-        const double wr = W[k * stride + 2 * e], wi = W[k * stride + 2 * e + 1];
+        const double wr = (double)W[k * stride + 2 * e], wi = (double)W[k * stride + 2 * e + 1];
         re[k] = fma(wr, xr, fma(wi, xi, re[k]));
         im[k] = fma(wr, xi, fma(-wi, xr, im[k]));
       }
@@ -1731,15 +1732,16 @@ k_eig_reduce_rows(const double* __restrict__ partial, int nb, double* __restrict
 }
 
 // out += sum_k c_k W_k with complex c_k (coef[2k], coef[2k+1]), k < K
+template <typename TW>
 __global__ void __launch_bounds__(256)
-k_eig_mcombo(double* __restrict__ out, const double* __restrict__ W, size_t stride, int K,
+k_eig_mcombo(double* __restrict__ out, const TW* __restrict__ W, size_t stride, int K,
              const double* __restrict__ coef, size_t ncx) {
   for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < ncx;
        e += (size_t)gridDim.x * blockDim.x) {
     double orr = out[2 * e], oi = out[2 * e + 1];
     for (int k = 0; k < K; ++k) {
       const double cr = coef[2 * k], ci = coef[2 * k + 1];
-      const double wr = W[k * stride + 2 * e], wi = W[k * stride + 2 * e + 1];
+      const double wr = (double)W[k * stride + 2 * e], wi = (double)W[k * stride + 2 * e + 1];
       orr = fma(cr, wr, fma(-ci, wi, orr));
       oi = fma(cr, wi, fma(ci, wr, oi));
     }
@@ -1757,6 +1759,63 @@ k_eig_round_f32(double* __restrict__ u, double* __restrict__ v, size_t n) {
     const double r = (double)(float)v[e];
     u[e] = r;
     v[e] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Implicitly restarted Lanczos (DiracOp::ImpResLanczos, d_op_base/comsrc/
+// lanczos.C:193-840): the Lanczos vectors are single precision, as in CPS.
+// ---------------------------------------------------------------------------
+constexpr int LZ_MAXM = 256;   // nk + np
+constexpr int LZ_TILE = 64;    // elements per work-group of k_lz_vq
+
+// V[k] = sum_{j<m} V[j] Q[j * m + k] for k < ncol, in place: a work-group
+// stages the m values of its 64 elements in LDS before it writes any
+__global__ void __launch_bounds__(LZ_TILE)
+k_lz_vq(float* __restrict__ V, size_t stride, size_t n, int m, int ncol,
+        const double* __restrict__ Q) {
+  extern __shared__ float sv[];  // [m][LZ_TILE]
+  const int t = threadIdx.x;
+  for (size_t base = (size_t)blockIdx.x * LZ_TILE; base < n; base += (size_t)gridDim.x * LZ_TILE) {
+    const size_t e = base + t;
+    const bool act = e < n;
+    for (int j = 0; j < m; ++j) sv[j * LZ_TILE + t] = act ? V[j * stride + e] : 0.0f;
+    __syncthreads();
+    if (act)
+      for (int k = 0; k < ncol; ++k) {
+        double acc = 0.0;
+        for (int j = 0; j < m; ++j) acc = fma((double)sv[j * LZ_TILE + t], Q[j * m + k], acc);
+        V[k * stride + e] = (float)acc;
+      }
+    __syncthreads();
+  }
+}
+
+// dst = (float)(a * src)
+__global__ void __launch_bounds__(256)
+k_lz_to_f32(float* __restrict__ dst, const double* __restrict__ src, double a, size_t n) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    dst[e] = (float)(a * src[e]);
+}
+// dst = src
+__global__ void __launch_bounds__(256)
+k_lz_to_f64(double* __restrict__ dst, const float* __restrict__ src, size_t n) {
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    dst[e] = (double)src[e];
+}
+// r -= c v with the complex c read from the device (c[0], c[1]): one step of
+// the Gram-Schmidt of a new Lanczos residual, no host round trip
+__global__ void __launch_bounds__(256)
+k_lz_gs_axpy(double* __restrict__ r, const float* __restrict__ v, const double* __restrict__ c,
+             size_t ncx) {
+  const double cr = c[0], ci = c[1];
+  for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < ncx;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const double vr = v[2 * e], vi = v[2 * e + 1];
+    r[2 * e] = r[2 * e] - (cr * vr - ci * vi);
+    r[2 * e + 1] = r[2 * e + 1] - (cr * vi + ci * vr);
   }
 }
 

@@ -48,7 +48,7 @@ EXPORTS = (
     "dwfgpu_fermion_from_canonical", "dwfgpu_fermion_to_canonical", "dwfgpu_four2five",
     "dwfgpu_five2four", "dwfgpu_matinv_dev", "dwfgpu_nersc_read", "dwfgpu_plaquette",
     "dwfgpu_inveigcg_dev", "dwfgpu_eigcg_create", "dwfgpu_eigcg_solve", "dwfgpu_eigcg_def_len",
-    "dwfgpu_eigcg_values", "dwfgpu_eigcg_destroy",
+    "dwfgpu_eigcg_values", "dwfgpu_eigcg_destroy", "dwfgpu_lanczos",
 )
 # every entry point of include/dwf_synth.h
 SYNTH_EXPORTS = ("dwfsynth_gauge", "dwfsynth_gaussian")
@@ -73,6 +73,13 @@ class NerscHeaderC(C.Structure):
                 ("big_endian", C.c_int), ("checksum", C.c_uint), ("checksum_data", C.c_uint),
                 ("plaquette", C.c_double), ("link_trace", C.c_double),
                 ("data_start", C.c_longlong)]
+
+
+class LanczosArgC(C.Structure):
+    _fields_ = [("mass", C.c_double), ("nk", C.c_int), ("nt", C.c_int), ("np", C.c_int),
+                ("maxiters", C.c_int), ("stop_residual", C.c_double), ("npol", C.c_int),
+                ("p0", C.c_double), ("p1", C.c_double), ("shift", C.c_double),
+                ("conv_check", C.c_int)]
 
 
 class CgArgC(C.Structure):
@@ -179,6 +186,8 @@ def lib():
         L.dwfgpu_eigcg_def_len.argtypes = [H]
         L.dwfgpu_eigcg_values.argtypes = [H, C.c_void_p, C.c_void_p]
         L.dwfgpu_eigcg_destroy.argtypes = [H]
+        L.dwfgpu_lanczos.argtypes = [H, C.POINTER(LanczosArgC), _dp, C.c_void_p, _dp,
+                                     C.POINTER(C.c_int)]
         L.dwfsynth_gauge.argtypes = [_dp, C.c_int * 4, C.c_double, C.c_ulonglong]
         L.dwfsynth_gauge.restype = None
         L.dwfsynth_gaussian.argtypes = [_dp, C.c_size_t, C.c_ulonglong]
@@ -541,6 +550,20 @@ class _DiracOpGpu:
             None if rs is None else rs.ctypes.data_as(C.c_void_p), 0 if rs is None else rs.size,
             C.byref(tr), C.byref(it)), "InvEigCg", allow=(OK, MAXITER))
         return it.value, tr.value, st, M[:2 * nev]
+
+    def ImpResLanczos(self, v0, mass, nk, nt, np_, maxiters, stop_residual, npol, p0, p1,
+                      shift=0.0, conv_check=1, vectors=True):
+        """DiracOp::ImpResLanczos on the Chebyshev-filtered MatPcDagMatPc.
+        Returns (lambda[nt], vectors [nt, n] or None, restarts)."""
+        a = LanczosArgC(mass, nk, nt, np_, maxiters, stop_residual, npol, p0, p1, shift,
+                        conv_check)
+        lam = np.zeros(nt)
+        ev = np.zeros(nt * self.n) if vectors else None
+        it = C.c_int()
+        _chk(lib().dwfgpu_lanczos(self.h, C.byref(a), _arr(v0),
+                                  None if ev is None else ev.ctypes.data_as(C.c_void_p), lam,
+                                  C.byref(it)), "ImpResLanczos")
+        return lam, (None if ev is None else ev.reshape(nt, self.n)), it.value
 
     def plaquette(self):
         """(SumReTrPlaq / (18 V), sum ReTr U / (12 V)) of the handle's links."""

@@ -342,6 +342,31 @@ int dwfgpu_eigcg_def_len(dwfgpu_eigcg_t e);
 int dwfgpu_eigcg_values(dwfgpu_eigcg_t e, double *M, double *H);
 int dwfgpu_eigcg_destroy(dwfgpu_eigcg_t e);
 
+/* Mirrors LanczosArg (include/input/eig_arg.x) for DiracOp::ImpResLanczos
+ * with RitzMat_lanczos = MATPCDAG_MATPC through the Chebyshev filter of
+ * DiracOpDwf::RitzMat (d_op_dwf.C:852-940) and RitzMat_convcheck =
+ * MATPCDAG_MATPC.                                                            */
+typedef struct {
+  double mass;
+  int nk, nt, np;       /* kept, wanted, extra Lanczos vectors (nk + np <= 256) */
+  int maxiters;         /* restarts                                           */
+  double stop_residual; /* |A v - <v, A v> v| / |v| of the nt wanted vectors   */
+  int npol;             /* Chebyshev degree (>= 1)                            */
+  double p0, p1, shift; /* filter: a = p0^2, b = (p1 + |shift|)^2             */
+  int conv_check;       /* check every conv_check restarts                    */
+} dwfgpu_lanczos_arg;
+/* DiracOp::ImpResLanczos (src/util/dirac_op/d_op_base/comsrc/lanczos.C:
+ * 193-840), FP64 handle: implicitly restarted Lanczos on the filtered
+ * MatPcDagMatPc with single-precision Lanczos vectors, exact shifts (the np
+ * smallest filtered Ritz values), a Gram-Schmidt of every new residual
+ * against all Lanczos vectors.  v0: the start vector (the CPS half vector
+ * passed in V[0]).  lambda[0..nt): Rayleigh quotients of the nt returned
+ * vectors (lowest modes of A for an even npol); evecs (host, nt*n reals, or
+ * NULL) receive the vectors, rounded to single precision as CPS keeps them.
+ * *iters = CPS's return value (restarts).                                   */
+int dwfgpu_lanczos(dwfgpu_t h, const dwfgpu_lanczos_arg *arg, const double *v0,
+                   double *evecs, double *lambda, int *iters);
+
 #ifdef __cplusplus
 }
 #endif
