@@ -72,6 +72,7 @@ struct PrecFields {
   void* U = nullptr;  // links (18 reals in FP64, first two rows otherwise)
   DVec te, t1;        // opposite-parity temporary, MatPc output inside MdagM
   DVec work[6];       // solver vectors, allocated on first use
+  std::vector<DVec> ring;  // CG directions of the last steps (mixed CG)
 };
 
 }  // namespace
@@ -237,7 +238,7 @@ size_t link_lds_bytes(const dwfgpu_s* h, int nsets, size_t creal) {
 // largest LDS request of any kernel of this handle (FP64 compute is the widest)
 size_t max_lds_bytes(const dwfgpu_s* h) {
   return h->action == DWFGPU_DWF ? link_lds_bytes(h, 2, 8)
-                                 : link_lds_bytes(h, 1, 8) + col_lds_bytes(h, 8);
+                                 : std::max(link_lds_bytes(h, 1, 8), col_lds_bytes(h, 8));
 }
 
 // ---------------------------------------------------------------------------
@@ -289,6 +290,16 @@ int ensure_work(dwfgpu_s* h, int count) {
   PrecFields& F = h->f[pidx<P>()];
   for (int k = 0; k < count; ++k)
     if (!F.work[k].d) TRY(alloc_vec<P>(h, &F.work[k]));
+  return DWFGPU_OK;
+}
+
+template <class P>
+int ensure_ring(dwfgpu_s* h, int count) {
+  TRY(ensure_fields<P>(h));
+  std::vector<DVec>& ring = h->f[pidx<P>()].ring;
+  if ((int)ring.size() < count) ring.resize(count);
+  for (int k = 0; k < count; ++k)
+    if (!ring[k].d) TRY(alloc_vec<P>(h, &ring[k]));
   return DWFGPU_OK;
 }
 
@@ -360,7 +371,8 @@ struct Ops {
     const MobArgs a = mob_args(h, mass, dag);
     const S* u = (const S*)F(h).U;
     const bool col = dag || epi == dwfk::EPI_M5INV;
-    const size_t lds = link_lds_bytes(h, 1, sizeof(R)) + (col ? col_lds_bytes(h, sizeof(R)) : 0);
+    const size_t lds =
+        std::max(link_lds_bytes(h, 1, sizeof(R)), col ? col_lds_bytes(h, sizeof(R)) : (size_t)0);
 #define MOB_LAUNCH(DAG, EPI, FL)                                                              \
   hipLaunchKernelGGL((dwfk::k_mobius<P, DAG, EPI, FL>), grid, block, lds, h->st, fld<P>(out), \
                      cfld<P>(in), cfld<P>(psi), u, h->partial, h->g, p_out, a, dwfk::CgEpi<P, P>{})
@@ -466,7 +478,7 @@ struct Ops {
       TRY(reduce_to(h, h->grid_hop, 1));
       TRY(mob_first(h, mass, te, t1, 1));
       const MobArgs a = mob_args(h, mass, 1);
-      const size_t lds = link_lds_bytes(h, 1, sizeof(R)) + col_lds_bytes(h, sizeof(R));
+      const size_t lds = std::max(link_lds_bytes(h, 1, sizeof(R)), col_lds_bytes(h, sizeof(R)));
       hipLaunchKernelGGL((dwfk::k_mobius<P, 1, dwfk::EPI_SCHUR, dwfk::FL_CG, PX>), grid, block,
                          lds, h->st, fld<P>(te), cfld<P>(te), cfld<P>(t1), u, h->partial, h->g, 1,
                          a, cg);
@@ -668,13 +680,18 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   typedef Ops<P64> H;
   typedef Ops<PS> L;
   typedef P32 XS;
+  // the directions of the last KRING steps stay in a ring of sloppy vectors;
+  // x_s += a_j p_j for all of them is one pass over x_s (k_x_flush) instead
+  // of one per step: the FP32 x_s moves through HBM once every KRING steps
+  constexpr int KRING = 8;
   TRY(ensure_work<P64>(h, 2));
-  TRY(ensure_work<PS>(h, 3));
+  TRY(ensure_work<PS>(h, 1));
   TRY(ensure_work<P32>(h, 4));
+  TRY(ensure_ring<PS>(h, KRING));
   const DVec& r = h->f[DWFGPU_F64].work[0];
   const DVec& ax = h->f[DWFGPU_F64].work[1];
   const DVec& rs = h->f[pidx<PS>()].work[0];
-  const DVec& ps = h->f[pidx<PS>()].work[1];
+  const std::vector<DVec>& ring = h->f[pidx<PS>()].ring;
   const DVec& xs = h->f[DWFGPU_F32].work[3];
   const double mass = arg->mass;
   double src_norm_sq;
@@ -688,9 +705,19 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   hipLaunchKernelGGL((dwfk::k_convert<PS, P64>), blas_grid(), blas_block(), 0, h->st, fld<PS>(rs),
                      cfld<P64>(r), h->g);
   HIPCHK(hipGetLastError());
-  TRY(L::copy(h, ps, rs));
+  int slot = 0;
+  TRY(L::copy(h, ring[0], rs));
   HIPCHK(hipMemsetAsync(xs.d, 0, h->n / 2 * sizeof(typename XS::S), h->st));
-  if (xs.sc) HIPCHK(hipMemsetAsync(xs.sc, 0, (size_t)h->g.vh * h->ls * sizeof(float), h->st));
+  dwfk::XFlush<PS> xf;
+  xf.cnt = 0;
+  auto flush = [&]() -> int {
+    if (xf.cnt == 0) return DWFGPU_OK;
+    hipLaunchKernelGGL((dwfk::k_x_flush<PS, XS>), blas_grid(), blas_block(), 0, h->st,
+                       fld<XS>(xs), xf, h->g);
+    HIPCHK(hipGetLastError());
+    xf.cnt = 0;
+    return DWFGPU_OK;
+  };
   double rr_update = rr;  // |r|^2 at the last reliable update
   int itr = 0, nrel = 0, status = DWFGPU_OK;
   const int max_itr = rr <= stp_cnd ? 0 : arg->max_num_iter - 1;
@@ -698,20 +725,21 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   while (!converged && itr < max_itr) {
     itr = itr + 1;
     double d, rr_new;
-    // dscal[0] holds rr; the fused step updates rs and dscal[0]; x_s += a p_s
-    // rides in the pass that forms the next direction
+    const DVec& ps = ring[slot];
+    // dscal[0] holds rr; the fused step updates rs and dscal[0] only
     TRY(L::template mdagm_cg<dwfk::NoX>(h, mass, ps, rs, DVec(), &rr_new, &d));
     if (d == 0.0) {
       status = DWFGPU_BREAKDOWN;
       break;
     }
-    const double a = rr / d;  // the alpha the hop epilogue used
-    bool reliable = false;
+    // the alpha the hop epilogue used; x_s += a p_s is pending
+    xf.p[xf.cnt] = (const typename PS::C*)ps.d;
+    xf.sc[xf.cnt] = ps.sc;
+    xf.a[xf.cnt] = rr / d;
+    xf.cnt++;
     if (rr_new <= stp_cnd || rr_new < delta * delta * rr_update) {
       // reliable update in FP64
-      reliable = true;
-      hipLaunchKernelGGL((dwfk::k_cg_xp<PS, XS, false>), blas_grid(), blas_block(), 0, h->st,
-                         fld<XS>(xs), fld<PS>(ps), cfld<PS>(rs), a, 0.0, h->g);
+      TRY(flush());
       hipLaunchKernelGGL((dwfk::k_accumulate<XS>), blas_grid(), blas_block(), 0, h->st,
                          fld<P64>(sol), fld<XS>(xs), h->g);
       HIPCHK(hipGetLastError());
@@ -729,19 +757,20 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
         break;
       }
     }
+    // p_s(next) = r_s + b p_s into the next ring slot; when every slot holds
+    // a pending direction the oldest is about to be overwritten: flush first
     const double b = rr_new / rr;
-    if (reliable) {
-      TRY(L::axpby(h, ps, rs, 1.0, b));  // p_s = r_s + b p_s
-    } else {
-      // x_s += a p_s ; p_s = r_s + b p_s
-      hipLaunchKernelGGL((dwfk::k_cg_xp<PS, XS, true>), blas_grid(), blas_block(), 0, h->st,
-                         fld<XS>(xs), fld<PS>(ps), cfld<PS>(rs), a, b, h->g);
-      HIPCHK(hipGetLastError());
-    }
+    const int next = (slot + 1) % KRING;
+    if (xf.cnt == KRING) TRY(flush());
+    hipLaunchKernelGGL((dwfk::k_pnext<PS>), blas_grid(), blas_block(), 0, h->st, fld<PS>(ring[next]),
+                       cfld<PS>(rs), cfld<PS>(ps), b, h->g);
+    HIPCHK(hipGetLastError());
+    slot = next;
     rr = rr_new;
   }
   if (!converged && status == DWFGPU_OK) {
     // fold the pending correction in and measure the true residual
+    TRY(flush());
     hipLaunchKernelGGL((dwfk::k_accumulate<XS>), blas_grid(), blas_block(), 0, h->st,
                        fld<P64>(sol), fld<XS>(xs), h->g);
     HIPCHK(hipGetLastError());
@@ -866,6 +895,8 @@ void free_all(dwfgpu_s* h) {
     free_vec(&F.te);
     free_vec(&F.t1);
     for (auto& w : F.work) free_vec(&w);
+    for (auto& w : F.ring) free_vec(&w);
+    F.ring.clear();
     F.ready = false;
   }
   void* ptrs[] = {h->cps_stage, h->stage4d, h->partial, h->dscal};

@@ -851,7 +851,11 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
   int* lnb = reinterpret_cast<int*>(smem_raw);                   // neighbours [8][ns]
   cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw + 8 * ns * 4);  // staged links [8][ns][9]
-  cx<R>* col = lu + 8 * ns * 9;                                  // s-columns [ns][sst]
+  // s-columns [ns][sst] reuse the LDS of the links and neighbour indices,
+  // which the hop no longer needs once every thread has passed the barrier
+  // before the first column write: the smaller footprint lets more
+  // work-groups share a CU
+  cx<R>* col = reinterpret_cast<cx<R>*>(smem_raw);
   const int sst = g.ls * 12 + 1;                    // padded site stride in LDS
   const int blk = xcd_block(g.nxcd);
   const int i = blk * ns + il;
@@ -912,6 +916,7 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
   }
 
   cx<R>* mycol = col + il * sst;
+  __syncthreads();  // every thread is done with the staged links
   if (DAG == 1) {
     // (b + c D5^dag) on the hop output: needs s-1 / s+1 of the same site
 #pragma unroll
@@ -1103,6 +1108,52 @@ k_cg_xp(Fld<PX> x, Fld<P> p, CFld<P> r, double a, double b, Geom g) {
       for (int c = 0; c < 12; ++c) pv[c] = rv[c] + br * pv[c];
       st_site_k<P>(p, g, k, pv);
     }
+  }
+}
+
+// pn = r + b p (the next CG direction into another buffer)
+template <class P>
+__global__ void __launch_bounds__(BLAS_THREADS)
+k_pnext(Fld<P> pn, CFld<P> r, CFld<P> p, double b, Geom g) {
+  typedef typename P::R R;
+  const R br = (R)b;
+  DWFK_SITE_LOOP(k, g) {
+    cx<R> rv[12], pv[12];
+    ld_site<P>(rv, r, g, k);
+    ld_site<P>(pv, p, g, k);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) pv[c] = rv[c] + br * pv[c];
+    st_site_k<P>(pn, g, k, pv);
+  }
+}
+
+// x += a_0 p_0, then += a_1 p_1, ... (the order and the rounding of one
+// update per CG step, k_cg_xp): the x updates of cnt steps in one pass
+constexpr int XFLUSH_MAX = 16;
+template <class P>
+struct XFlush {
+  const typename P::C* p[XFLUSH_MAX];
+  const float* sc[XFLUSH_MAX];
+  double a[XFLUSH_MAX];
+  int cnt;
+};
+template <class P, class PX>
+__global__ void __launch_bounds__(BLAS_THREADS)
+k_x_flush(Fld<PX> x, XFlush<P> f, Geom g) {
+  typedef typename P::R R;
+  typedef typename PX::R RX;
+  DWFK_SITE_LOOP(k, g) {
+    cx<RX> xv[12];
+    ld_site<PX>(xv, CFld<PX>{x.d, x.sc}, g, k);
+    for (int j = 0; j < f.cnt; ++j) {
+      cx<R> pv[12];
+      ld_site<P>(pv, CFld<P>{f.p[j], f.sc[j]}, g, k);
+      const double a = f.a[j];
+#pragma unroll
+      for (int c = 0; c < 12; ++c)
+        xv[c] = cx<RX>{(RX)(a * pv[c].re) + xv[c].re, (RX)(a * pv[c].im) + xv[c].im};
+    }
+    st_site_k<PX>(x, g, k, xv);
   }
 }
 
