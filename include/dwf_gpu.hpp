@@ -108,6 +108,50 @@ class DiracOpGpu {
     if (reliable_updates) *reliable_updates = nrel;
     return iters;
   }
+  // DiracOp::MInvCG (d_op_base/noarch/minvcg.C:47-355): psi = nshift vectors
+  // (MULTI) or one (SINGLE, with alpha); rsdcg in: relative stop residuals,
+  // out: |r_s|; returns CPS's loop count k
+  int MInvCG(double* psi, const double* chi, const double* shifts, int nshift, double* rsdcg,
+             int type = DWFGPU_SHIFT_MULTI, const double* alpha = nullptr,
+             int* iters = nullptr, double chi_norm = 0.0) {
+    int k = 0;
+    check(dwfgpu_minvcg(h_, arg_.mass, shifts, nshift, psi, chi, chi_norm > 0 ? chi_norm : -1.0,
+                        rsdcg, type, alpha, arg_.max_num_iter, iters, &k),
+          "MInvCG", true);
+    return k;
+  }
+  // DiracOp::ImpResLanczos (d_op_base/comsrc/lanczos.C:193-840) on the
+  // Chebyshev-filtered MatPcDagMatPc: lambda[nt], evecs nt half vectors (or
+  // nullptr); returns the restart count
+  int ImpResLanczos(const double* v0, double* lambda, double* evecs,
+                    const dwfgpu_lanczos_arg& a) {
+    int it = 0;
+    dwfgpu_lanczos_arg arg = a;
+    arg.mass = arg_.mass;
+    check(dwfgpu_lanczos(h_, &arg, v0, evecs, lambda, &it), "ImpResLanczos");
+    return it;
+  }
+  // QPropW's 4D propagator column (QPropW.C:1204-1236): Ffour2five(src4,
+  // 0, Ls-1) -> MatInv -> Ffive2four(sol4, Ls-1, 0), all 5D work on the
+  // device; src4 / sol4 are CANONICAL 4D fermions (V4*24 reals)
+  int PropagatorColumn(double* sol4, const double* src4, int ls, double* true_res = nullptr) {
+    dwfgpu_vec_t v[4] = {nullptr, nullptr, nullptr, nullptr};
+    int st = DWFGPU_OK;
+    for (auto& x : v)
+      if (!st) st = dwfgpu_vec_create(h_, &x);
+    int iters = 0;
+    const dwfgpu_cg_arg a = arg_.c();
+    if (!st) st = dwfgpu_four2five(h_, v[0], v[1], src4, 0, ls - 1);
+    if (!st) st = dwfgpu_matinv_dev(h_, &a, v[2], v[3], v[0], v[1], true_res, &iters);
+    if (st == DWFGPU_OK || st == DWFGPU_MAXITER) {
+      const int r = dwfgpu_five2four(h_, sol4, v[2], v[3], ls - 1, 0);
+      if (r) st = r;
+    }
+    for (auto& x : v)
+      if (x) dwfgpu_vec_destroy(x);
+    check(st, "PropagatorColumn", true);
+    return iters;
+  }
   // DiracOp::DiracArg(CgArg*): the mass and CG controls of later calls
   void DiracArg(const CgArg& arg) { arg_ = arg; }
 

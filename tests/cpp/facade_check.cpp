@@ -3,7 +3,9 @@
 //   MatPcDag(MatPc(x)) == MatPcDagMatPc(x) bit for bit,
 //   InvCg through the facade == dwfgpu_invcg (same iterations, same bits),
 //   |src - A sol| / |src| recomputed here == the returned true residual,
-//   MAXITER is reported as a count, a bad argument as dwfgpu::Error.
+//   MAXITER is reported as a count, a bad argument as dwfgpu::Error;
+//   MInvCG solves every shift, ImpResLanczos returns eigenpairs, the
+//   Ffour2five -> MatInv -> Ffive2four column converges.
 // Prints "facade_check OK" and exits 0 on success.
 #include <cmath>
 #include <cstdio>
@@ -80,7 +82,51 @@ static int run(const char* name) {
     threw = e.code() == DWFGPU_BAD_ARG;
   }
   REQUIRE(threw);
-  std::printf("%s: %d CG iterations, true residual %.3e\n", name, it1, tr1);
+  op.DiracArg(arg);
+
+  // MInvCG (MULTI): every shift solved, (A + sigma) psi_s = chi
+  const double shifts[3] = {0.0, 0.01, 0.1};
+  double rsd[3] = {1e-10, 1e-10, 1e-10};
+  std::vector<double> psi(3 * n, 0.0);
+  int its[3] = {0, 0, 0};
+  const int k = op.MInvCG(psi.data(), x.data(), shifts, 3, rsd, DWFGPU_SHIFT_MULTI, nullptr, its);
+  REQUIRE(k > 0 && its[0] >= its[1] && its[1] >= its[2]);
+  for (int j = 0; j < 3; ++j) {
+    op.MatPcDagMatPc(a.data(), psi.data() + j * n);
+    for (size_t q = 0; q < n; ++q) a[q] = x[q] - a[q] - shifts[j] * psi[j * n + q];
+    REQUIRE(std::sqrt(norm2(a) / norm2(x)) < 1e-9);
+  }
+
+  // ImpResLanczos: the lowest pairs are eigenpairs of MatPcDagMatPc
+  dwfgpu_lanczos_arg la{};
+  la.nk = 8;
+  la.nt = 2;
+  la.np = 8;
+  la.maxiters = 100;
+  la.stop_residual = 1e-6;
+  la.npol = 6;
+  la.p0 = 2.2;  // above the spectrum of MatPcDagMatPc here
+  la.p1 = 0.4;
+  la.conv_check = 1;
+  double lam[2];
+  std::vector<double> ev(2 * n);
+  REQUIRE(op.ImpResLanczos(x.data(), lam, ev.data(), la) >= 1);
+  for (int j = 0; j < 2; ++j) {
+    op.MatPcDagMatPc(a.data(), ev.data() + j * n);
+    std::vector<double> v(ev.begin() + j * n, ev.begin() + (j + 1) * n);
+    for (size_t q = 0; q < n; ++q) a[q] -= lam[j] * v[q];
+    REQUIRE(lam[j] > 0 && std::sqrt(norm2(a) / norm2(v)) < 1e-4 * lam[j] + 1e-6);
+  }
+
+  // a 4D point source through Ffour2five -> MatInv -> Ffive2four
+  std::vector<double> src4(v4 * 24, 0.0), sol4(v4 * 24, 0.0);
+  src4[2 * 4] = 1.0;
+  double ptr = 1;
+  const int pit = op.PropagatorColumn(sol4.data(), src4.data(), p.ls, &ptr);
+  REQUIRE(pit > 0 && ptr <= 1e-10 && norm2(sol4) > 0);
+  std::printf("%s: %d CG iterations, true residual %.3e; MInvCG k=%d; lowest eigenvalues "
+              "%.6e %.6e; propagator column %d iterations\n",
+              name, it1, tr1, k, lam[0], lam[1], pit);
   return 0;
 }
 
