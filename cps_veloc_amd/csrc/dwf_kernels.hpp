@@ -640,13 +640,19 @@ struct NoX {
   static constexpr int kLinkElems = 9;
 };
 
+// alpha = rr / <p, A p> from the device scalars.  The kernels read it at
+// their start: loaded and divided next to the epilogue, the compiler issued
+// every neighbour load of the hop ahead of it and spilled them to scratch.
+__device__ __forceinline__ double cg_alpha(const double* scal) {
+  const double rr = scal[0], d = scal[1];
+  return d != 0.0 ? rr / d : 0.0;
+}
+
 template <class P, class PX>
 __device__ __forceinline__ double cg_epilogue(const CgEpi<P, PX>& cg, const Geom& g, int s,
-                                              int i, const cx<typename P::R>* ap) {
+                                              int i, const cx<typename P::R>* ap, double a) {
   typedef typename P::R R;
   typedef typename PX::R RX;
-  const double rr = cg.scal[0], d = cg.scal[1];
-  const double a = d != 0.0 ? rr / d : 0.0;
   const size_t sl = (size_t)s * nch<P>() * g.vh, slx = (size_t)s * nch<PX>() * g.vh;
   const size_t ss = (size_t)s * g.vh;
   cx<R> r[12];
@@ -685,6 +691,7 @@ k_dwf_dslash(Fld<P> out, CFld<P> in, const typename P::S* __restrict__ U, CFld<P
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
   int* lnb = reinterpret_cast<int*>(smem_raw);                   // [2][8][ns]
   cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw + 2 * 8 * ns * 4);  // [2][8][ns][9]
+  const double a_cg = CG ? cg_alpha(cg.scal) : 0.0;
   const int blk = xcd_block(g.nxcd);
   const int i = blk * ns + il;
   // slice s holds output parity (s + 1 + cb_in) % 2: set s % 2 of the stage
@@ -721,7 +728,7 @@ k_dwf_dslash(Fld<P> out, CFld<P> in, const typename P::S* __restrict__ U, CFld<P
 #pragma unroll
       for (int c = 0; c < 12; ++c) acc[c] = xv[c] + a * acc[c];
       if (NORM) nrm = norm12(acc);
-      if (CG) nrm = cg_epilogue<P, PX>(cg, g, s, i, acc);
+      if (CG) nrm = cg_epilogue<P, PX>(cg, g, s, i, acc, a_cg);
     }
     if (!CG) st_site<P>(out.d + s * slice, out.sc + (size_t)s * g.vh, g.vh, i, acc);
   }
@@ -857,6 +864,7 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
   // work-groups share a CU
   cx<R>* col = reinterpret_cast<cx<R>*>(smem_raw);
   const int sst = g.ls * 12 + 1;                    // padded site stride in LDS
+  const double a_cg = (FL & FL_CG) ? cg_alpha(cg.scal) : 0.0;
   const int blk = xcd_block(g.nxcd);
   const int i = blk * ns + il;
   const bool act = i < g.vh;
@@ -936,7 +944,7 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
 #pragma unroll
         for (int c = 0; c < 12; ++c) v[c] = v[c] + tv[c];
         if (FL & FL_CG)
-          nrm = cg_epilogue<P, PX>(cg, g, s, i, v);
+          nrm = cg_epilogue<P, PX>(cg, g, s, i, v, a_cg);
         else
           st_site<P>(od, osc, g.vh, i, v);
       }
@@ -1580,6 +1588,7 @@ __global__ void __launch_bounds__(512) k_dwf_mrhs(MrArgs<P, PX> a) {
   stage_links<P>(lu, lnb, a.U, g, blk, ns, 2, (1 + a.cb_in) & 1);
   __syncthreads();
   const bool act = j < a.nrhs && a.active[j] && i < g.vh;
+  const double a_cg = (CG && j < a.nrhs) ? cg_alpha(a.scal + (size_t)j * MR_NSCAL) : 0.0;
   const size_t vss = (size_t)g.vh * g.ls;
   double nrm = 0.0;
   if (act) {
@@ -1617,7 +1626,7 @@ __global__ void __launch_bounds__(512) k_dwf_mrhs(MrArgs<P, PX> a) {
         const CgEpi<P, PX> cg{fld_at(a.r, a.vs, vss, j), fld_at(a.p, a.vs, vss, j),
                               fld_at(a.x, a.vs * nch<PX>() / nch<P>(), vss, j),
                               a.scal + (size_t)j * MR_NSCAL};
-        nrm = cg_epilogue<P, PX>(cg, g, s, i, acc);
+        nrm = cg_epilogue<P, PX>(cg, g, s, i, acc, a_cg);
       }
     }
     if (!CG) {
