@@ -89,6 +89,7 @@ struct dwfgpu_s {
   PrecFields f[NPREC];
   void* cps_stage = nullptr;  // FP64 buffer in CPS order (host-array API)
   double* stage4d = nullptr;  // one 4D CANONICAL fermion slice (format calls)
+  int* nbt = nullptr;         // neighbour table of the hop kernels (g.nbt)
   double* partial = nullptr;
   int npartial = 0;
   double* dscal = nullptr;
@@ -210,6 +211,14 @@ M5Side m5_side(const dwfgpu_s* h, double mass, int dag) {
     }
     k.inv_f = inv_d.real();
   }
+  for (int s = 0; s < dwfk::MAXLS; ++s) {
+    k.faf[s] = (float)k.af[s];
+    k.fcf[s] = (float)k.cf[s];
+    k.fab[s] = (float)k.ab[s];
+    k.fcbk[s] = (float)k.cbk[s];
+  }
+  k.finv_f = (float)k.inv_f;
+  k.finv_b = (float)k.inv_b;
   return k;
 }
 
@@ -222,6 +231,12 @@ MobArgs mob_args(const dwfgpu_s* h, double mass, int dag) {
   a.K = kb / kc;
   a.mK = -mass * a.K;
   a.kappa_b = kb;
+  a.fb = (float)a.b;
+  a.fc = (float)a.c;
+  a.fmc = (float)a.mc;
+  a.fK = (float)a.K;
+  a.fmK = (float)a.mK;
+  a.fkappa_b = (float)a.kappa_b;
   a.m5 = m5_side(h, mass, dag);
   return a;
 }
@@ -899,7 +914,7 @@ void free_all(dwfgpu_s* h) {
     F.ring.clear();
     F.ready = false;
   }
-  void* ptrs[] = {h->cps_stage, h->stage4d, h->partial, h->dscal};
+  void* ptrs[] = {h->cps_stage, h->stage4d, h->partial, h->dscal, h->nbt};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->hscal) (void)hipHostFree(h->hscal);
@@ -2726,6 +2741,13 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
     delete h;
     return DWFGPU_HIP_ERROR;
   }
+  if ((r = alloc(h, (void**)&h->nbt, (size_t)2 * 8 * h->g.vh * sizeof(int)))) return fail(r);
+  hipLaunchKernelGGL(dwfk::k_build_nbt, dim3(2048), dim3(256), 0, h->st, h->nbt, h->g);
+  if (hipGetLastError() != hipSuccess) {
+    set_err("dwfgpu_create: neighbour table kernel failed to launch");
+    return fail(DWFGPU_HIP_ERROR);
+  }
+  h->g.nbt = h->nbt;
   if ((r = upload_gauge(h, gauge))) return fail(r);
   if ((r = ensure_api(h, 0))) return fail(r);
   if ((r = alloc(h, &h->cps_stage, h->n * sizeof(double)))) return fail(r);

@@ -186,10 +186,13 @@ struct Pack<PI16> {
   __device__ static cx<float> unpack(uint32_t v, float sc) {
     return {(float)(int16_t)(v & 0xffffu) * sc, (float)(int16_t)(v >> 16) * sc};
   }
+  // inv = 1 / (largest |component| of the site): the signed-normalised
+  // conversion clamps to [-1, 1], scales by 32767 and rounds to nearest, two
+  // components per instruction
   __device__ static uint32_t pack(cx<float> v, float inv) {
-    const int a = max(-32767, min(32767, __float2int_rn(v.re * inv)));
-    const int b = max(-32767, min(32767, __float2int_rn(v.im * inv)));
-    return ((uint32_t)a & 0xffffu) | ((uint32_t)b << 16);
+    typedef short short2_t __attribute__((ext_vector_type(2)));
+    const short2_t q = __builtin_amdgcn_cvt_pknorm_i16(v.re * inv, v.im * inv);
+    return ((uint32_t)(uint16_t)q.x) | ((uint32_t)(uint16_t)q.y << 16);
   }
 };
 // fixed scale of compressed 16-bit links
@@ -272,9 +275,9 @@ __device__ __forceinline__ void st_site(typename P::C* __restrict__ d, float* __
     float m = 0.0f;
 #pragma unroll
     for (int c = 0; c < 12; ++c) m = fmaxf(m, fmaxf(fabsf((float)v[c].re), fabsf((float)v[c].im)));
-    const float s = scale_of<P>(m);
-    inv = m > 0.0f ? 1.0f / s : 0.0f;
-    sc[n] = s;
+    // the packers take 1 / m (bf16: value / scale; i16: normalised to [-1, 1])
+    inv = m > 0.0f ? 1.0f / m : 0.0f;
+    sc[n] = scale_of<P>(m);
   }
 #pragma unroll
   for (int k = 0; k < nch<P>(); ++k) d[(size_t)k * vh + n] = chunk_pack<P>(v + k * P::kCPC, inv);
@@ -288,6 +291,11 @@ struct Geom {
   int bt, bz;  // time / z slices per tile of the site index (divide lt / lz)
   int nxcd;    // XCDs the work-groups are spread over
   int bcmask;  // bit mu set: anti-periodic sign on the links leaving x_mu = L_mu - 1
+  // neighbour table of the staged hop, nbt[(p * 8 + d) * vh + i] for output
+  // site i of parity p: the site index of the neighbour x + d (d < 4) or
+  // x - (d - 4), bit 31 set when the link between them carries the
+  // anti-periodic sign (built once by k_build_nbt)
+  const int* nbt;
 };
 
 // site index of (x/2, y, z, t): order (fastest first) x/2, t%bt, z%bz, y,
@@ -488,6 +496,25 @@ __device__ __forceinline__ void link_site(const Geom& g, const int* c, int i, bo
 // site x; `set` distinguishes the two output parities of a DWF work-group.
 // lnb[(set*8 + d)*ns + site] receives the index of the neighbour x + d
 // (d < 4) or x - (d-4) that the hop reads, so no slice recomputes it.
+// one entry of the neighbour table (layout: Geom::nbt)
+__global__ void __launch_bounds__(256) k_build_nbt(int* __restrict__ nbt, Geom g) {
+  const int n = 2 * 8 * g.vh;
+  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
+    const int i = w % g.vh, d = (w / g.vh) % 8, p = w / (8 * g.vh);
+    int c[4];
+    site_coords(g, i, p, c);
+    int site = i, nbr = i;
+    bool neg = false;
+    switch (d & 3) {
+      case 0: link_site<0>(g, c, i, d >= 4, &site, &neg, &nbr); break;
+      case 1: link_site<1>(g, c, i, d >= 4, &site, &neg, &nbr); break;
+      case 2: link_site<2>(g, c, i, d >= 4, &site, &neg, &nbr); break;
+      default: link_site<3>(g, c, i, d >= 4, &site, &neg, &nbr); break;
+    }
+    nbt[w] = (int)((unsigned)nbr | (neg ? 0x80000000u : 0u));
+  }
+}
+
 template <class P>
 __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
                                             int* __restrict__ lnb,
@@ -503,18 +530,10 @@ __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
     if (i >= g.vh) continue;
     const int p = (p_first + set) & 1;
     const int mu = d & 3;
-    int c[4];
-    site_coords(g, i, p, c);
-    int site = i, nbr = i;
-    bool neg = false;
-    // one case per direction: a lane-dependent index into {lx, ly, lz, lt}
-    // would be lowered to a private (scratch) array
-    switch (mu) {
-      case 0: link_site<0>(g, c, i, d >= 4, &site, &neg, &nbr); break;
-      case 1: link_site<1>(g, c, i, d >= 4, &site, &neg, &nbr); break;
-      case 2: link_site<2>(g, c, i, d >= 4, &site, &neg, &nbr); break;
-      default: link_site<3>(g, c, i, d >= 4, &site, &neg, &nbr); break;
-    }
+    const int e = g.nbt[(size_t)(p * 8 + d) * g.vh + i];
+    const int nbr = e & 0x7fffffff;
+    const bool neg = e < 0;
+    const int site = d >= 4 ? nbr : i;
     lnb[w] = nbr;
     const int par = d >= 4 ? 1 - p : p;  // U_mu(x - mu) lives on the other parity
     const typename P::S* src = U + (size_t)((mu * 2 + par) * NE) * g.vh + site;
@@ -753,50 +772,99 @@ enum { EPI_M5INV = 0, EPI_SCHUR = 1, EPI_PLAIN = 2 };
 
 // One chirality's M5^-1 (or M5^-dag) along s: coefficients of the two-sweep
 // recurrences of zmobius_m5inv.C:247-396 / :534-679, real for real b, c.
+// Coefficients are kept in FP64 and FP32 so that neither compute type
+// converts them per use.
 struct M5Side {
   double af[MAXLS];   // fwd-corner sweep: v[s] += af[s] v[s-1]
   double cf[MAXLS];   // fwd-corner correction: v[s] += cf[s] v[ls-1]
   double ab[MAXLS];   // bwd-corner sweep: v[s] += ab[s] v[s+1]
   double cbk[MAXLS];  // bwd-corner gather: v[ls-1] += cbk[s] v[s]
   double inv_f, inv_b;
+  float faf[MAXLS], fcf[MAXLS], fab[MAXLS], fcbk[MAXLS];
+  float finv_f, finv_b;
 };
 
 struct MobArgs {
   double b, c, mc;  // Mobius b, c and -mass*c (wall coefficient)
   double K, mK;     // kappa_b/kappa_c and -mass*K
   double kappa_b;
+  float fb, fc, fmc, fK, fmK, fkappa_b;
   M5Side m5;
 };
 
+// the coefficient in the compute type R
+template <typename R>
+__device__ __forceinline__ R pick(double d, float f) {
+  if constexpr (sizeof(R) == 4)
+    return f;
+  else
+    return d;
+}
+
 template <typename R>
 __device__ __forceinline__ void m5_fwd_corner(cx<R>* v, int ls, const M5Side& k) {
-  for (int s = 1; s < ls; ++s) v[s * 12] = v[s * 12] + (R)k.af[s] * v[(s - 1) * 12];
+  for (int s = 1; s < ls; ++s)
+    v[s * 12] = v[s * 12] + pick<R>(k.af[s], k.faf[s]) * v[(s - 1) * 12];
   const cx<R> zl = v[(ls - 1) * 12];
-  for (int s = 0; s < ls - 1; ++s) v[s * 12] = v[s * 12] + (R)k.cf[s] * zl;
-  v[(ls - 1) * 12] = (R)k.inv_f * zl;
+  for (int s = 0; s < ls - 1; ++s) v[s * 12] = v[s * 12] + pick<R>(k.cf[s], k.fcf[s]) * zl;
+  v[(ls - 1) * 12] = pick<R>(k.inv_f, k.finv_f) * zl;
 }
 
 template <typename R>
 __device__ __forceinline__ void m5_bwd_corner(cx<R>* v, int ls, const M5Side& k) {
-  cx<R> last = (R)k.inv_b * v[(ls - 1) * 12];
-  for (int s = 0; s < ls - 1; ++s) last = last + (R)k.cbk[s] * v[s * 12];
+  cx<R> last = pick<R>(k.inv_b, k.finv_b) * v[(ls - 1) * 12];
+  for (int s = 0; s < ls - 1; ++s) last = last + pick<R>(k.cbk[s], k.fcbk[s]) * v[s * 12];
   v[(ls - 1) * 12] = last;
-  for (int s = ls - 2; s >= 0; --s) v[s * 12] = v[s * 12] + (R)k.ab[s] * v[(s + 1) * 12];
+  for (int s = ls - 2; s >= 0; --s)
+    v[s * 12] = v[s * 12] + pick<R>(k.ab[s], k.fab[s]) * v[(s + 1) * 12];
+}
+
+// the same on a column held in registers (LS known at compile time)
+template <int LS, int DAG, typename R>
+__device__ __forceinline__ void m5inv_column_reg(cx<R>* v, bool upper, const M5Side& k) {
+  cx<R> r[LS];
+#pragma unroll
+  for (int s = 0; s < LS; ++s) r[s] = v[s * 12];
+  if (upper == (DAG == 0)) {
+#pragma unroll
+    for (int s = 1; s < LS; ++s) r[s] = r[s] + pick<R>(k.af[s], k.faf[s]) * r[s - 1];
+    const cx<R> zl = r[LS - 1];
+#pragma unroll
+    for (int s = 0; s < LS - 1; ++s) r[s] = r[s] + pick<R>(k.cf[s], k.fcf[s]) * zl;
+    r[LS - 1] = pick<R>(k.inv_f, k.finv_f) * zl;
+  } else {
+    cx<R> last = pick<R>(k.inv_b, k.finv_b) * r[LS - 1];
+#pragma unroll
+    for (int s = 0; s < LS - 1; ++s) last = last + pick<R>(k.cbk[s], k.fcbk[s]) * r[s];
+    r[LS - 1] = last;
+#pragma unroll
+    for (int s = LS - 2; s >= 0; --s) r[s] = r[s] + pick<R>(k.ab[s], k.fab[s]) * r[s + 1];
+  }
+#pragma unroll
+  for (int s = 0; s < LS; ++s) v[s * 12] = r[s];
 }
 
 // M5^-1 (dag 0: P+ forward-corner, P- backward-corner; dag 1 swapped) on the
-// s-columns staged in LDS; one thread per (site, complex component).
+// s-columns staged in LDS; one thread per (site, complex component).  Columns
+// are numbered component-major, so that whole wavefronts take the same
+// corner; for the common Ls the column is swept in registers.
 template <int DAG, typename R>
 __device__ __forceinline__ void m5inv_columns(cx<R>* col, int sst, int ns, int nvalid, int ls,
                                               const M5Side& k) {
   const int tid = threadIdx.x + blockDim.x * threadIdx.y;
   const int nthr = blockDim.x * blockDim.y;
   for (int w = tid; w < ns * 12; w += nthr) {
-    const int sl = w / 12, cc = w % 12;
+    const int cc = w / ns, sl = w - cc * ns;
     if (sl >= nvalid) continue;
     cx<R>* v = col + sl * sst + cc;
     const bool upper = cc < 6;
-    if (upper == (DAG == 0))
+    if (ls == 12)
+      m5inv_column_reg<12, DAG>(v, upper, k);
+    else if (ls == 16)
+      m5inv_column_reg<16, DAG>(v, upper, k);
+    else if (ls == 8)
+      m5inv_column_reg<8, DAG>(v, upper, k);
+    else if (upper == (DAG == 0))
       m5_fwd_corner(v, ls, k);
     else
       m5_bwd_corner(v, ls, k);
@@ -819,8 +887,9 @@ __device__ __forceinline__ void apply_t_col(cx<R>* tv, const cx<R>* col_site, in
   const cx<R>* h0 = col_site + s * 12;
   const cx<R>* hm = col_site + (s > 0 ? s - 1 : ls - 1) * 12;
   const cx<R>* hp = col_site + (s < ls - 1 ? s + 1 : 0) * 12;
-  const R cm = (R)(s > 0 ? a.c : a.mc), cp = (R)(s < ls - 1 ? a.c : a.mc);
-  const R b = (R)a.b;
+  const R cm = s > 0 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
+  const R cp = s < ls - 1 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
+  const R b = pick<R>(a.b, a.fb);
 #pragma unroll
   for (int c = 0; c < 12; ++c) {
     const bool from_minus = (c < 6) == (dag == 0);
@@ -836,7 +905,8 @@ __device__ __forceinline__ void m5_term(cx<typename P::R>* v, CFld<P> psi, const
   const size_t slice = (size_t)nch<P>() * g.vh;
   const int ls = g.ls;
   const int sm = s > 0 ? s - 1 : ls - 1, sp = s < ls - 1 ? s + 1 : 0;
-  const R km = (R)(s > 0 ? a.K : a.mK), kp = (R)(s < ls - 1 ? a.K : a.mK);
+  const R km = s > 0 ? pick<R>(a.K, a.fK) : pick<R>(a.mK, a.fmK);
+  const R kp = s < ls - 1 ? pick<R>(a.K, a.fK) : pick<R>(a.mK, a.fmK);
   cx<R> vm[12], vp[12];
   ld_spinor<P>(psi.d + s * slice, psi.sc + (size_t)s * g.vh, g.vh, i, v);
   ld_spinor<P>(psi.d + sm * slice, psi.sc + (size_t)sm * g.vh, g.vh, i, vm);
@@ -890,9 +960,9 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
       ld.sc_prv = in.sc + (size_t)sm * g.vh;
       ld.sc_nxt = in.sc + (size_t)sp * g.vh;
       ld.vh = g.vh;
-      ld.b = (R)a.b;
-      ld.cm = (R)(s > 0 ? a.c : a.mc);
-      ld.cp = (R)(s < ls - 1 ? a.c : a.mc);
+      ld.b = pick<R>(a.b, a.fb);
+      ld.cm = s > 0 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
+      ld.cp = s < ls - 1 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
       hop4<0>(acc, ld, lu + il * 9, ns * 9, lnb + il, ns);
     } else {
       hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
@@ -910,7 +980,7 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
     if (act) {
       cx<R> v[12];
       m5_term<P>(v, psi, g, s, i, a, 0);
-      const R nkb = (R)(-a.kappa_b);
+      const R nkb = -pick<R>(a.kappa_b, a.fkappa_b);
 #pragma unroll
       for (int c = 0; c < 12; ++c) v[c] = v[c] + nkb * acc[c];
       if (FL & FL_NORM) nrm = norm12(v);
@@ -955,11 +1025,11 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
       return;
     }
     __syncthreads();
-    const R nkb = (R)(-a.kappa_b);
+    const R nkb = -pick<R>(a.kappa_b, a.fkappa_b);
 #pragma unroll
     for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = nkb * tv[c];
   } else {
-    const R kb = (R)a.kappa_b;
+    const R kb = pick<R>(a.kappa_b, a.fkappa_b);
 #pragma unroll
     for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = kb * acc[c];
   }
@@ -971,7 +1041,7 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
     if (DAG == 0 && (FL & FL_POST_T)) {
       apply_t_col(v, mycol, s, ls, a, 0);
     } else {
-      const R sc = (R)(DAG ? a.kappa_b : 1.0);
+      const R sc = DAG ? pick<R>(a.kappa_b, a.fkappa_b) : (R)1;
 #pragma unroll
       for (int c = 0; c < 12; ++c) v[c] = sc * mycol[s * 12 + c];
     }
@@ -1438,7 +1508,7 @@ k_compress_links(typename P::S* __restrict__ dst, const cx<double>* __restrict__
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x) {
     const size_t mp = k / g.vh, i = k - mp * g.vh;
-    const float inv = 1.0f / link_scale<P>();
+    const float inv = 1.0f;  // |link element| <= 1: bf16 as is, i16 normalised
 #pragma unroll
     for (int e = 0; e < 6; ++e) {
       const cx<double> u = src[(mp * 9 + e) * g.vh + i];
