@@ -385,16 +385,33 @@ struct Ops {
     const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const MobArgs a = mob_args(h, mass, dag);
     const S* u = (const S*)F(h).U;
-    const bool col = dag || epi == dwfk::EPI_M5INV;
-    const size_t lds =
-        std::max(link_lds_bytes(h, 1, sizeof(R)), col ? col_lds_bytes(h, sizeof(R)) : (size_t)0);
+    const bool col = dag || epi == dwfk::EPI_M5INV || epi == dwfk::EPI_SCHUR;
+    const size_t split_lds = dwfk::split_col_offset<R>(h->ns) + col_lds_bytes(h, sizeof(R));
+    const bool split = dwfk::MobSplitT<P>::value && !dag && !(fl & dwfk::FL_PRE_T) &&
+                       split_lds <= (size_t)h->max_lds;
+    const size_t lds = split ? split_lds
+                             : std::max(link_lds_bytes(h, 1, sizeof(R)),
+                                        col ? col_lds_bytes(h, sizeof(R)) : (size_t)0);
 #define MOB_LAUNCH(DAG, EPI, FL)                                                              \
   hipLaunchKernelGGL((dwfk::k_mobius<P, DAG, EPI, FL>), grid, block, lds, h->st, fld<P>(out), \
                      cfld<P>(in), cfld<P>(psi), u, h->partial, h->g, p_out, a, dwfk::CgEpi<P, P>{})
     using dwfk::FL_NORM;
     using dwfk::FL_POST_T;
     using dwfk::FL_PRE_T;
-    if (dag == 0 && epi == dwfk::EPI_M5INV) {
+    if (split) {
+      if constexpr (dwfk::MobSplitT<P>::value) {
+        using dwfk::FL_SPLIT_T;
+        if (epi == dwfk::EPI_M5INV) {
+          if (fl & FL_POST_T) MOB_LAUNCH(0, dwfk::EPI_M5INV, FL_POST_T | FL_SPLIT_T);
+          else MOB_LAUNCH(0, dwfk::EPI_M5INV, FL_SPLIT_T);
+        } else if (epi == dwfk::EPI_SCHUR) {
+          if (fl & FL_NORM) MOB_LAUNCH(0, dwfk::EPI_SCHUR, FL_NORM | FL_SPLIT_T);
+          else MOB_LAUNCH(0, dwfk::EPI_SCHUR, FL_SPLIT_T);
+        } else {
+          MOB_LAUNCH(0, dwfk::EPI_PLAIN, FL_SPLIT_T);
+        }
+      }
+    } else if (dag == 0 && epi == dwfk::EPI_M5INV) {
       if (fl & FL_POST_T) MOB_LAUNCH(0, dwfk::EPI_M5INV, FL_POST_T);
       else MOB_LAUNCH(0, dwfk::EPI_M5INV, 0);
     } else if (dag == 0 && epi == dwfk::EPI_SCHUR) {
@@ -686,7 +703,7 @@ struct Ops {
 // sloppy residual meets the stop condition; the solve ends only when the
 // FP64 true residual meets it (stop rule of inv_cg.C:174,354).
 // The correction x_s is kept in FP32 for every sloppy policy: an int16 x_s
-// (per-site scale) rounds each step's increment to 1/32767 of the site's
+// (per-site scale) rounds each step's increment to about 2^-15 of the site's
 // accumulated correction, which on the 24^3x64 Mobius system raised the
 // iteration count from 525 to 661.
 template <class PS>

@@ -151,8 +151,15 @@ template <class P>
 __device__ __forceinline__ float scale_of(float) { return 1.0f; }
 template <>
 __device__ __forceinline__ float scale_of<PBF16>(float m) { return m; }
+// int16 components are stored in [-16383, 16383] (site maximum -> 16383), so
+// that the hop can add two of them in int16 before converting
 template <>
-__device__ __forceinline__ float scale_of<PI16>(float m) { return m * (1.0f / 32767.0f); }
+__device__ __forceinline__ float scale_of<PI16>(float m) { return m * (1.0f / 16383.0f); }
+// factor of 1 / m handed to the packer: pknorm maps [-1, 1] to +-32767
+template <class P>
+__device__ __forceinline__ float pack_norm() { return 1.0f; }
+template <>
+__device__ __forceinline__ float pack_norm<PI16>() { return 16383.0f / 32767.0f; }
 
 __device__ __forceinline__ uint32_t bf16_rne(float x) {
   uint32_t b = __float_as_uint(x);
@@ -186,9 +193,9 @@ struct Pack<PI16> {
   __device__ static cx<float> unpack(uint32_t v, float sc) {
     return {(float)(int16_t)(v & 0xffffu) * sc, (float)(int16_t)(v >> 16) * sc};
   }
-  // inv = 1 / (largest |component| of the site): the signed-normalised
-  // conversion clamps to [-1, 1], scales by 32767 and rounds to nearest, two
-  // components per instruction
+  // v * inv in [-1, 1] (inv from st_site, or 1 for links): the
+  // signed-normalised conversion clamps to [-1, 1], scales by 32767 and
+  // rounds to nearest, two components per instruction
   __device__ static uint32_t pack(cx<float> v, float inv) {
     typedef short short2_t __attribute__((ext_vector_type(2)));
     const short2_t q = __builtin_amdgcn_cvt_pknorm_i16(v.re * inv, v.im * inv);
@@ -257,8 +264,8 @@ __device__ __forceinline__ void ld_spinor(const typename P::C* __restrict__ d,
   for (int k = 0; k < nch<P>(); ++k) chunk_unpack<P>(d[(size_t)k * vh + n], s, v + k * P::kCPC);
 }
 
-// the same without the per-site scale (16-bit policies: raw values / 32767
-// or raw bf16; the caller applies the scale)
+// the same without the per-site scale (16-bit policies: the raw integers or
+// raw bf16; the caller applies the scale)
 template <class P>
 __device__ __forceinline__ void ld_spinor_raw(const typename P::C* __restrict__ d, int vh, int n,
                                               cx<typename P::R>* v) {
@@ -275,8 +282,9 @@ __device__ __forceinline__ void st_site(typename P::C* __restrict__ d, float* __
     float m = 0.0f;
 #pragma unroll
     for (int c = 0; c < 12; ++c) m = fmaxf(m, fmaxf(fabsf((float)v[c].re), fabsf((float)v[c].im)));
-    // the packers take 1 / m (bf16: value / scale; i16: normalised to [-1, 1])
-    inv = m > 0.0f ? 1.0f / m : 0.0f;
+    // the packers take 1 / m (bf16: value / scale; i16: times pack_norm,
+    // so that the site maximum lands on 16383)
+    inv = m > 0.0f ? pack_norm<P>() / m : 0.0f;
     sc[n] = scale_of<P>(m);
   }
 #pragma unroll
@@ -422,19 +430,95 @@ __device__ __forceinline__ void reconstruct(cx<R>* a, const cx<R>* r, R w) {
   }
 }
 
-// plain load of the input spinor at site n of one slice
-// A loader fills p with the spinor at site n; for 16-bit policies it leaves
-// out the per-site scale and returns it, so the caller folds it into the
-// accumulation of the hop's product (no separate multiplications).
+// (1 + SG gamma_mu) on int16 components packed (re | im << 16), two lanes
+// per instruction.  Stored int16 magnitudes are at most 16383, so sums and
+// differences stay inside int16.
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+template <int SG>
+__device__ __forceinline__ i16x2 pk_addi(i16x2 a, i16x2 b) {  // a + SG i b
+  // (re, im) of i b = (-b.im, b.re): b with its halves swapped (op_sel)
+  // times (-1, 1), plus a, in one packed multiply-add (the low 16 bits of
+  // a product do not depend on signedness)
+  const i16x2 k = SG > 0 ? i16x2{-1, 1} : i16x2{1, -1};
+  i16x2 r;
+  asm("v_pk_mad_u16 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[0,1,1]"
+      : "=v"(r)
+      : "v"(b), "v"(k), "v"(a));
+  return r;
+}
+template <int SG>
+__device__ __forceinline__ i16x2 pk_addr(i16x2 a, i16x2 b) {  // a + SG b
+  return SG > 0 ? a + b : a - b;
+}
+template <int MU, int SG>
+__device__ __forceinline__ void project_pk(const uint32_t* w, cx<float>* h) {
+  i16x2 p[12], q[6];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) p[c] = __builtin_bit_cast(i16x2, w[c]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    if (MU == 0) {
+      q[c] = pk_addi<SG>(p[c], p[9 + c]);
+      q[3 + c] = pk_addi<SG>(p[3 + c], p[6 + c]);
+    } else if (MU == 1) {
+      q[c] = pk_addr<-SG>(p[c], p[9 + c]);
+      q[3 + c] = pk_addr<SG>(p[3 + c], p[6 + c]);
+    } else if (MU == 2) {
+      q[c] = pk_addi<SG>(p[c], p[6 + c]);
+      q[3 + c] = pk_addi<-SG>(p[3 + c], p[9 + c]);
+    } else {
+      q[c] = pk_addr<SG>(p[c], p[6 + c]);
+      q[3 + c] = pk_addr<SG>(p[3 + c], p[9 + c]);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) h[c] = cx<float>{(float)q[c].x, (float)q[c].y};
+}
+
+// the 12 packed words of an int16 spinor (complex c in word c)
+__device__ __forceinline__ void ld_words_i16(const uint4* __restrict__ d, int vh, int n,
+                                             uint32_t* w) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const uint4 u = d[(size_t)k * vh + n];
+    w[4 * k] = u.x;
+    w[4 * k + 1] = u.y;
+    w[4 * k + 2] = u.z;
+    w[4 * k + 3] = u.w;
+  }
+}
+
+// A loader gives the hop the projected two-spinor (1 + SG gamma_mu) psi of
+// the input at site n; for 16-bit policies it leaves out the per-site scale
+// and returns it, so the caller folds it into the accumulation of the hop's
+// product (no separate multiplications).
 template <class P>
 struct PlainLoader {
+  typedef typename P::R R;
   const typename P::C* __restrict__ d;
   const float* __restrict__ sc;
   int vh;
-  __device__ __forceinline__ float operator()(int n, cx<typename P::R>* p) const {
+  // the 12 raw components (scale returned, not applied)
+  __device__ __forceinline__ float load(int n, cx<R>* p) const {
 #pragma unroll
     for (int k = 0; k < nch<P>(); ++k)
       chunk_unpack<P>(d[(size_t)k * vh + n], 1.0f, p + k * P::kCPC);
+    return ld_scale<P>(sc, n);
+  }
+  template <int MU, int SG>
+  __device__ __forceinline__ float proj(int n, cx<R>* h) const {
+    if constexpr (std::is_same<P, PI16>::value) {
+      // project the raw integers, then convert 6 complex instead of 12
+      uint32_t w[12];
+      ld_words_i16(d, vh, n, w);
+      project_pk<MU, SG>(w, h);
+    } else {
+      cx<R> p[12];
+#pragma unroll
+      for (int k = 0; k < nch<P>(); ++k)
+        chunk_unpack<P>(d[(size_t)k * vh + n], 1.0f, p + k * P::kCPC);
+      project<MU, SG>(p, h);
+    }
     return ld_scale<P>(sc, n);
   }
 };
@@ -452,23 +536,47 @@ struct MobTLoader {
   const float* __restrict__ sc_nxt;
   int vh;
   R b, cm, cp;
-  __device__ __forceinline__ float operator()(int n, cx<R>* p) const {
-    cx<R> vm[12], vp[12];
+  template <int MU, int SG>
+  __device__ __forceinline__ float proj(int n, cx<R>* h) const {
+    cx<R> vm[12], vp[12], q[12];
     // raw values: the three per-site scales go into the coefficients
     // (3 products per site instead of one per component)
-    ld_spinor_raw<P>(cur, vh, n, p);
-    ld_spinor_raw<P>(prv, vh, n, vm);
-    ld_spinor_raw<P>(nxt, vh, n, vp);
-    R bs = b, cms = cm, cps = cp;
-    if constexpr (P::kScaled) {
-      bs = b * sc_cur[n];
-      cms = cm * sc_prv[n];
-      cps = cp * sc_nxt[n];
+    if constexpr (std::is_same<P, PI16>::value) {
+      // b proj(cur) + proj(c P+ prv + c P- nxt): cur projected in int16
+      // (half the conversions); the second term takes the upper spins of prv
+      // and the lower spins of nxt, one product each
+      uint32_t w[12];
+      cx<R> hq[6];
+      ld_words_i16(cur, vh, n, w);
+      project_pk<MU, SG>(w, h);
+      ld_spinor_raw<P>(prv, vh, n, vm);
+      ld_spinor_raw<P>(nxt, vh, n, vp);
+      const R bs = b * sc_cur[n], cms = cm * sc_prv[n], cps = cp * sc_nxt[n];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) q[c] = cms * vm[c];
+#pragma unroll
+      for (int c = 6; c < 12; ++c) q[c] = cps * vp[c];
+      project<MU, SG>(q, hq);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) h[c] = bs * h[c] + hq[c];
+    } else {
+      // t first, then its projection (the FP32 / FP64 kernels keep their
+      // register budget this way)
+      ld_spinor_raw<P>(cur, vh, n, q);
+      ld_spinor_raw<P>(prv, vh, n, vm);
+      ld_spinor_raw<P>(nxt, vh, n, vp);
+      R bs = b, cms = cm, cps = cp;
+      if constexpr (P::kScaled) {
+        bs = b * sc_cur[n];
+        cms = cm * sc_prv[n];
+        cps = cp * sc_nxt[n];
+      }
+#pragma unroll
+      for (int c = 0; c < 6; ++c) q[c] = bs * q[c] + cms * vm[c];
+#pragma unroll
+      for (int c = 6; c < 12; ++c) q[c] = bs * q[c] + cps * vp[c];
+      project<MU, SG>(q, h);
     }
-#pragma unroll
-    for (int c = 0; c < 6; ++c) p[c] = bs * p[c] + cms * vm[c];
-#pragma unroll
-    for (int c = 6; c < 12; ++c) p[c] = bs * p[c] + cps * vp[c];
     return 1.0f;
   }
 };
@@ -560,10 +668,9 @@ template <int MU, int DAG, typename R, class LD>
 __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* lf,
                                         const cx<R>* lb, const Nbr& nb) {
   constexpr int SF = DAG ? 1 : -1;  // forward: 1 - s gamma, s = +1 (dag 0) / -1 (dag 1)
-  cx<R> psi[12], h[6], r[6];
+  cx<R> h[6], r[6];
   {
-    const R sc = (R)ld(nb.fwd[MU], psi);
-    project<MU, SF>(psi, h);
+    const R sc = (R)ld.template proj<MU, SF>(nb.fwd[MU], h);
     cx<R> m[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) m[e] = lf[e];
@@ -576,8 +683,7 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
     reconstruct<MU, SF>(acc, r, sc);
   }
   {
-    const R sc = (R)ld(nb.bwd[MU], psi);
-    project<MU, -SF>(psi, h);
+    const R sc = (R)ld.template proj<MU, -SF>(nb.bwd[MU], h);
     cx<R> m[9];
 #pragma unroll
     for (int e = 0; e < 9; ++e) m[e] = lb[e];
@@ -589,6 +695,65 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
                                   h[3 * k + 1], h[3 * k + 2]);
     reconstruct<MU, -SF>(acc, r, sc);
   }
+}
+
+// The hop of one chirality of its input: acc += hop(P+ psi) (UP) or
+// hop(P- psi).  The projection of P+ psi is its upper two spins as they are,
+// that of P- psi the sign/phase-mapped lower two, so each pass reads half of
+// every neighbour spinor (the unused chunk loads vanish).
+template <int MU, int SG, bool FWD, typename R>
+__device__ __forceinline__ void hop_half(cx<R>* acc, const cx<R>* h, const cx<R>* l, R sc) {
+  cx<R> r[6];
+#pragma unroll
+  for (int k = 0; k < 2; ++k)
+#pragma unroll
+    for (int row = 0; row < 3; ++row)
+      r[3 * k + row] = FWD ? row_dot(l[row * 3 + 0], l[row * 3 + 1], l[row * 3 + 2], h[3 * k + 0],
+                                     h[3 * k + 1], h[3 * k + 2])
+                           : row_dotc(l[0 * 3 + row], l[1 * 3 + row], l[2 * 3 + row],
+                                      h[3 * k + 0], h[3 * k + 1], h[3 * k + 2]);
+  reconstruct<MU, SG>(acc, r, sc);
+}
+
+template <int MU, int SG, bool UP, typename R, class LD>
+__device__ __forceinline__ void chiral_proj(const LD& ld, int n, cx<R>* h, R* sc) {
+  cx<R> p[12];
+  *sc = (R)ld.load(n, p);
+  if (UP) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) h[c] = p[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) p[c] = cx<R>{0, 0};
+    project<MU, SG>(p, h);
+  }
+}
+
+template <int MU, int DAG, bool UP, typename R, class LD>
+__device__ __forceinline__ void hop_dir_chiral(cx<R>* acc, const LD& ld, const cx<R>* lf,
+                                               const cx<R>* lb, const Nbr& nb) {
+  constexpr int SF = DAG ? 1 : -1;
+  cx<R> h[6];
+  R sc;
+  chiral_proj<MU, SF, UP>(ld, nb.fwd[MU], h, &sc);
+  hop_half<MU, SF, true>(acc, h, lf, sc);
+  chiral_proj<MU, -SF, UP>(ld, nb.bwd[MU], h, &sc);
+  hop_half<MU, -SF, false>(acc, h, lb, sc);
+}
+
+template <int DAG, bool UP, typename R, class LD>
+__device__ __forceinline__ void hop4_chiral(cx<R>* acc, const LD& ld, const cx<R>* lu, int ds,
+                                            const int* nbi, int ns) {
+  Nbr nb;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    nb.fwd[mu] = nbi[mu * ns];
+    nb.bwd[mu] = nbi[(4 + mu) * ns];
+  }
+  hop_dir_chiral<0, DAG, UP>(acc, ld, lu + 0 * ds, lu + 4 * ds, nb);
+  hop_dir_chiral<1, DAG, UP>(acc, ld, lu + 1 * ds, lu + 5 * ds, nb);
+  hop_dir_chiral<2, DAG, UP>(acc, ld, lu + 2 * ds, lu + 6 * ds, nb);
+  hop_dir_chiral<3, DAG, UP>(acc, ld, lu + 3 * ds, lu + 7 * ds, nb);
 }
 
 // acc = D_hop^(dag) psi at output site i of parity p_out (wilson_dslash.C:173-517);
@@ -872,13 +1037,15 @@ __device__ __forceinline__ void m5inv_columns(cx<R>* col, int sst, int ns, int n
 }
 
 // FL flags of k_mobius
-enum { FL_NORM = 1, FL_PRE_T = 2, FL_POST_T = 4, FL_CG = 8 };
+enum { FL_NORM = 1, FL_PRE_T = 2, FL_POST_T = 4, FL_CG = 8, FL_SPLIT_T = 16 };
 // FL_NORM   : |out|^2 partials (dag 0 EPI_SCHUR)
 // FL_PRE_T  : dag 0, `in` already holds (b + c D5) applied to the operand, so
 //             the gather is a plain load
 // FL_POST_T : dag 0 EPI_M5INV, write (b + c D5) M5^-1 (...) so that the next
 //             pass can use FL_PRE_T
 // FL_CG     : dag 1 EPI_SCHUR, feed the result to cg_epilogue instead of `out`
+// FL_SPLIT_T: dag 0 without FL_PRE_T, gather (b + c D5) in by chirality
+//             (two half-spinor passes, LDS after the links: split_col_offset)
 template <typename R>
 __device__ __forceinline__ void apply_t_col(cx<R>* tv, const cx<R>* col_site, int s, int ls,
                                             const MobArgs& a, int dag) {
@@ -897,25 +1064,36 @@ __device__ __forceinline__ void apply_t_col(cx<R>* tv, const cx<R>* col_site, in
   }
 }
 
-// psi(s) + K D5^(dag) psi (s) at site i (the Schur "M5" term, zmobius_m-orig.h:96-121)
-template <class P>
-__device__ __forceinline__ void m5_term(cx<typename P::R>* v, CFld<P> psi, const Geom& g, int s,
-                                        int i, const MobArgs& a, int dag) {
-  typedef typename P::R R;
-  const size_t slice = (size_t)nch<P>() * g.vh;
-  const int ls = g.ls;
-  const int sm = s > 0 ? s - 1 : ls - 1, sp = s < ls - 1 ? s + 1 : 0;
+// psi(s) + K D5^(dag) psi (s), the Schur "M5" term (zmobius_m-orig.h:96-121),
+// with psi(s) already in v and the s-column of psi in LDS, so that psi is
+// read from memory once per (site, slice) instead of three times
+template <typename R>
+__device__ __forceinline__ void m5_term_col(cx<R>* v, const cx<R>* col_site, int s, int ls,
+                                            const MobArgs& a, int dag) {
+  const cx<R>* hm = col_site + (s > 0 ? s - 1 : ls - 1) * 12;
+  const cx<R>* hp = col_site + (s < ls - 1 ? s + 1 : 0) * 12;
   const R km = s > 0 ? pick<R>(a.K, a.fK) : pick<R>(a.mK, a.fmK);
   const R kp = s < ls - 1 ? pick<R>(a.K, a.fK) : pick<R>(a.mK, a.fmK);
-  cx<R> vm[12], vp[12];
-  ld_spinor<P>(psi.d + s * slice, psi.sc + (size_t)s * g.vh, g.vh, i, v);
-  ld_spinor<P>(psi.d + sm * slice, psi.sc + (size_t)sm * g.vh, g.vh, i, vm);
-  ld_spinor<P>(psi.d + sp * slice, psi.sc + (size_t)sp * g.vh, g.vh, i, vp);
 #pragma unroll
   for (int c = 0; c < 12; ++c) {
     const bool from_minus = (c < 6) == (dag == 0);
-    v[c] = v[c] + (from_minus ? km * vm[c] : kp * vp[c]);
+    v[c] = v[c] + (from_minus ? km * hm[c] : kp * hp[c]);
   }
+}
+
+// whether the dag-0 gather of (b + c D5) in is split by chirality (each
+// neighbour read once, twice the SU(3) products) rather than reading the
+// slices s-1, s, s+1 of every neighbour: FP64, where the kernel waits on
+// memory rather than on arithmetic
+template <class P>
+struct MobSplitT {
+  static constexpr bool value = std::is_same<P, P64>::value;
+};
+// byte offset of the s-column of the split gather: after the staged
+// neighbour indices and links, 16-byte aligned
+template <typename R>
+__host__ __device__ constexpr size_t split_col_offset(int ns) {
+  return ((size_t)8 * ns * 4 + (size_t)8 * ns * 9 * 2 * sizeof(R) + 15) / 16 * 16;
 }
 
 template <class P, int DAG, int EPI, int FL, class PX = P>
@@ -949,7 +1127,38 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
   cx<R> acc[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
-  if (act) {
+  constexpr bool split_t = DAG == 0 && !(FL & FL_PRE_T) && (FL & FL_SPLIT_T);
+  cx<R>* mycol = col + il * sst;
+  if constexpr (split_t) {
+    // hop((b + c D5) in)(s) = b (A + B)(s) + c A(s-1) + c B(s+1) with
+    // A = hop(P+ in), B = hop(P- in) (the hop acts on every slice alike).
+    // Two passes over the directions, each reading one half of every
+    // neighbour spinor: the neighbours are read once in all, instead of
+    // the slices s-1, s, s+1 of each.  A waits in an LDS s-column placed
+    // after the staged links (the links serve both passes).
+    const PlainLoader<P> ld{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh};
+    cx<R>* cola = reinterpret_cast<cx<R>*>(smem_raw + split_col_offset<R>(ns)) + il * sst;
+    if (act) hop4_chiral<0, true>(acc, ld, lu + il * 9, ns * 9, lnb + il, ns);
+#pragma unroll
+    for (int c = 0; c < 12; ++c) cola[s * 12 + c] = acc[c];
+    cx<R> accb[12];
+#pragma unroll
+    for (int c = 0; c < 12; ++c) accb[c] = cx<R>{0, 0};
+    if (act) hop4_chiral<0, false>(accb, ld, lu + il * 9, ns * 9, lnb + il, ns);
+    const int sm = s > 0 ? s - 1 : ls - 1, sp = s < ls - 1 ? s + 1 : 0;
+    const R b = pick<R>(a.b, a.fb);
+    const R cm = s > 0 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
+    const R cp = s < ls - 1 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
+    __syncthreads();  // A of every slice is in LDS
+#pragma unroll
+    for (int c = 0; c < 12; ++c) acc[c] = b * (acc[c] + accb[c]) + cm * cola[sm * 12 + c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 12; ++c) cola[s * 12 + c] = accb[c];
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 12; ++c) acc[c] = acc[c] + cp * cola[sp * 12 + c];
+  } else if (act) {
     if (DAG == 0 && !(FL & FL_PRE_T)) {
       MobTLoader<P> ld;
       const int sm = s > 0 ? s - 1 : ls - 1, sp = s < ls - 1 ? s + 1 : 0;
@@ -976,10 +1185,17 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
   }
 
   if (DAG == 0 && EPI == EPI_SCHUR) {
+    // psi(s) through the s-column (the links' LDS is free after the barrier)
+    cx<R> v[12];
+    if (act) ld_spinor<P>(psi.d + s * slice, psi.sc + (size_t)s * g.vh, g.vh, i, v);
+    __syncthreads();
+    if (act)
+#pragma unroll
+      for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = v[c];
+    __syncthreads();
     double nrm = 0.0;
     if (act) {
-      cx<R> v[12];
-      m5_term<P>(v, psi, g, s, i, a, 0);
+      m5_term_col(v, mycol, s, ls, a, 0);
       const R nkb = -pick<R>(a.kappa_b, a.fkappa_b);
 #pragma unroll
       for (int c = 0; c < 12; ++c) v[c] = v[c] + nkb * acc[c];
@@ -993,7 +1209,6 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
     return;
   }
 
-  cx<R>* mycol = col + il * sst;
   __syncthreads();  // every thread is done with the staged links
   if (DAG == 1) {
     // (b + c D5^dag) on the hop output: needs s-1 / s+1 of the same site
@@ -1007,10 +1222,17 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
       return;
     }
     if (EPI == EPI_SCHUR) {
+      // psi(s) through the s-column, once every thread has read the hop's
+      cx<R> v[12];
+      if (act) ld_spinor<P>(psi.d + s * slice, psi.sc + (size_t)s * g.vh, g.vh, i, v);
+      __syncthreads();
+      if (act)
+#pragma unroll
+        for (int c = 0; c < 12; ++c) mycol[s * 12 + c] = v[c];
+      __syncthreads();
       double nrm = 0.0;
       if (act) {
-        cx<R> v[12];
-        m5_term<P>(v, psi, g, s, i, a, 1);
+        m5_term_col(v, mycol, s, ls, a, 1);
 #pragma unroll
         for (int c = 0; c < 12; ++c) v[c] = v[c] + tv[c];
         if (FL & FL_CG)

@@ -41,8 +41,8 @@ extern "C" {
  * F32, BF16 and I16 store links as their first two rows (the third is
  * rebuilt as the conjugate cross product times the boundary sign); BF16 and I16
  * store each (site, slice) spinor as 24 16-bit numbers with one FP32 scale
- * (BF16: value = bf16 * scale, I16: value = int16 * scale / 32767), and
- * compute in FP32.                                                          */
+ * (BF16: value = bf16 * scale; I16: value = int16 * scale, |int16| <= 16383
+ * with the site maximum at 16383), and compute in FP32.                     */
 #define DWFGPU_F64 0
 #define DWFGPU_F32 1
 #define DWFGPU_BF16 2
