@@ -715,7 +715,7 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   // the directions of the last KRING steps stay in a ring of sloppy vectors;
   // x_s += a_j p_j for all of them is one pass over x_s (k_x_flush) instead
   // of one per step: the FP32 x_s moves through HBM once every KRING steps
-  constexpr int KRING = 8;
+  constexpr int KRING = 16;
   TRY(ensure_work<P64>(h, 2));
   TRY(ensure_work<PS>(h, 1));
   TRY(ensure_work<P32>(h, 4));
