@@ -93,7 +93,9 @@ struct dwfgpu_s {
   double* partial = nullptr;
   int npartial = 0;
   double* dscal = nullptr;
-  double* hscal = nullptr;  // pinned
+  double* hscal = nullptr;   // pinned
+  double* dalpha = nullptr;  // alphas of the pending sloppy CG steps (device)
+  double* hring = nullptr;   // pinned copies of the sloppy CG scalars, per step
   int grid_hop = 0;
   int ns = 16;  // sites per work-group of the hopping kernels (x); y = all slices
   int max_lds = 65536;
@@ -485,10 +487,12 @@ struct Ops {
   // One CG step around MatPcDagMatPc (inv_cg.C:318-349) with the vector
   // updates fused into the last kernel: with rr = dscal[0] on entry,
   // d = <p, A p> -> dscal[1], a = rr / d, x += a p, r -= a A p, and
-  // |r|^2 -> dscal[0].  *rr_new, *d receive the two scalars.
+  // |r|^2 -> dscal[0] (mdagm_cg; *rr_new, *d receive the two scalars).
+  // mdagm_cg_launch issues the kernels only: the |r|^2 partials are left in
+  // h->partial (grid_hop of them) and <p, A p> in dscal[1].
   template <class PX>
-  static int mdagm_cg(dwfgpu_s* h, double mass, const DVec& p, const DVec& r, const DVec& x,
-                      double* rr_new, double* d) {
+  static int mdagm_cg_launch(dwfgpu_s* h, double mass, const DVec& p, const DVec& r,
+                             const DVec& x) {
     const DVec& te = F(h).te;
     const DVec& t1 = F(h).t1;
     const dwfk::CgEpi<P, PX> cg{fld<P>(r), cfld<P>(p), fld<PX>(x), h->dscal};
@@ -516,6 +520,13 @@ struct Ops {
                          a, cg);
     }
     HIPCHK(hipGetLastError());
+    return DWFGPU_OK;
+  }
+
+  template <class PX = P>
+  static int mdagm_cg(dwfgpu_s* h, double mass, const DVec& p, const DVec& r, const DVec& x,
+                      double* rr_new, double* d) {
+    TRY(mdagm_cg_launch<PX>(h, mass, p, r, x));
     TRY(reduce_to(h, h->grid_hop, 0));
     double sc[2];
     TRY(fetch_scalars(h, 2, sc));
@@ -742,6 +753,7 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   HIPCHK(hipMemsetAsync(xs.d, 0, h->n / 2 * sizeof(typename XS::S), h->st));
   dwfk::XFlush<PS> xf;
   xf.cnt = 0;
+  xf.alpha = h->dalpha;
   auto flush = [&]() -> int {
     if (xf.cnt == 0) return DWFGPU_OK;
     hipLaunchKernelGGL((dwfk::k_x_flush<PS, XS>), blas_grid(), blas_block(), 0, h->st,
@@ -750,55 +762,84 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
     xf.cnt = 0;
     return DWFGPU_OK;
   };
+  // The host examines the scalars of step k while the GPU runs steps k+1 and
+  // k+2 (alpha, beta and the x_s updates take their scalars on the device),
+  // so the GPU never waits for the host between steps.  A reliable update is
+  // therefore taken LAG steps after its condition first holds, with the
+  // state of the current step.
+  constexpr int LAG = 2, NEV = 4;
+  struct Events {
+    hipEvent_t e[NEV] = {};
+    ~Events() {
+      for (auto& x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } ev;
+  for (auto& x : ev.e) HIPCHK(hipEventCreateWithFlags(&x, hipEventDisableTiming));
   double rr_update = rr;  // |r|^2 at the last reliable update
   int itr = 0, nrel = 0, status = DWFGPU_OK;
+  int fresh_from = 1;  // steps before this one were covered by a reliable update
   const int max_itr = rr <= stp_cnd ? 0 : arg->max_num_iter - 1;
   bool converged = rr <= stp_cnd;
   while (!converged && itr < max_itr) {
     itr = itr + 1;
-    double d, rr_new;
     const DVec& ps = ring[slot];
-    // dscal[0] holds rr; the fused step updates rs and dscal[0] only
-    TRY(L::template mdagm_cg<dwfk::NoX>(h, mass, ps, rs, DVec(), &rr_new, &d));
-    if (d == 0.0) {
-      status = DWFGPU_BREAKDOWN;
-      break;
-    }
-    // the alpha the hop epilogue used; x_s += a p_s is pending
+    // dscal[0] holds |r_s|^2; the fused step updates r_s, and
+    // k_reduce_cg_step the scalars and the step's alpha (x_s += a p_s pending)
+    TRY(L::template mdagm_cg_launch<dwfk::NoX>(h, mass, ps, rs, DVec()));
+    hipLaunchKernelGGL(dwfk::k_reduce_cg_step, dim3(1), dim3(1024), 0, h->st, h->partial,
+                       h->grid_hop, h->dscal, h->dalpha + slot);
+    HIPCHK(hipGetLastError());
+    double* hs = h->hring + 4 * (itr % NEV);
+    HIPCHK(hipMemcpyAsync(hs, h->dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipEventRecord(ev.e[itr % NEV], h->st));
     xf.p[xf.cnt] = (const typename PS::C*)ps.d;
     xf.sc[xf.cnt] = ps.sc;
-    xf.a[xf.cnt] = rr / d;
+    xf.ai[xf.cnt] = slot;
     xf.cnt++;
-    if (rr_new <= stp_cnd || rr_new < delta * delta * rr_update) {
-      // reliable update in FP64
+    // the scalars of step itr - LAG (or of this step once max_itr is reached)
+    const int k = itr == max_itr ? itr : itr - LAG;
+    bool update = false;
+    if (k >= fresh_from) {
+      HIPCHK(hipEventSynchronize(ev.e[k % NEV]));
+      const double rr_k = h->hring[4 * (k % NEV)], d_k = h->hring[4 * (k % NEV) + 1];
+      if (d_k == 0.0) {
+        status = DWFGPU_BREAKDOWN;
+        break;
+      }
+      update = rr_k <= stp_cnd || rr_k < delta * delta * rr_update;
+    }
+    if (update) {
+      // reliable update in FP64 (after step itr)
       TRY(flush());
       hipLaunchKernelGGL((dwfk::k_accumulate<XS>), blas_grid(), blas_block(), 0, h->st,
                          fld<P64>(sol), fld<XS>(xs), h->g);
       HIPCHK(hipGetLastError());
       TRY(H::mdagm(h, mass, ax, sol, nullptr));
       TRY(H::axpy_to(h, r, ax, src, -1.0, true));
-      TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr_new));
+      double rr_new;
+      TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr_new));  // into dscal[0]: beta uses it
       hipLaunchKernelGGL((dwfk::k_convert<PS, P64>), blas_grid(), blas_block(), 0, h->st,
                          fld<PS>(rs), cfld<P64>(r), h->g);
       HIPCHK(hipGetLastError());
       rr_update = rr_new;
       ++nrel;
+      fresh_from = itr + 1;
       if (rr_new <= stp_cnd) {
         rr = rr_new;
         converged = true;
         break;
       }
     }
-    // p_s(next) = r_s + b p_s into the next ring slot; when every slot holds
-    // a pending direction the oldest is about to be overwritten: flush first
-    const double b = rr_new / rr;
+    // p_s(next) = r_s + b p_s into the next ring slot (b from the device
+    // scalars); when every slot holds a pending direction the oldest is
+    // about to be overwritten: flush first
     const int next = (slot + 1) % KRING;
     if (xf.cnt == KRING) TRY(flush());
     hipLaunchKernelGGL((dwfk::k_pnext<PS>), blas_grid(), blas_block(), 0, h->st, fld<PS>(ring[next]),
-                       cfld<PS>(rs), cfld<PS>(ps), b, h->g);
+                       cfld<PS>(rs), cfld<PS>(ps), h->dscal, h->g);
     HIPCHK(hipGetLastError());
     slot = next;
-    rr = rr_new;
   }
   if (!converged && status == DWFGPU_OK) {
     // fold the pending correction in and measure the true residual
@@ -931,10 +972,11 @@ void free_all(dwfgpu_s* h) {
     F.ring.clear();
     F.ready = false;
   }
-  void* ptrs[] = {h->cps_stage, h->stage4d, h->partial, h->dscal, h->nbt};
+  void* ptrs[] = {h->cps_stage, h->stage4d, h->partial, h->dscal, h->nbt, h->dalpha};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (h->hscal) (void)hipHostFree(h->hscal);
+  if (h->hring) (void)hipHostFree(h->hring);
   if (h->st) (void)hipStreamDestroy(h->st);
 }
 
@@ -2770,6 +2812,11 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
   if ((r = alloc(h, &h->cps_stage, h->n * sizeof(double)))) return fail(r);
   if ((r = alloc(h, (void**)&h->partial, h->npartial * sizeof(double)))) return fail(r);
   if ((r = alloc(h, (void**)&h->dscal, 8 * sizeof(double)))) return fail(r);
+  if ((r = alloc(h, (void**)&h->dalpha, 32 * sizeof(double)))) return fail(r);
+  if (hipHostMalloc((void**)&h->hring, 64 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
+    set_err("dwfgpu_create: hipHostMalloc failed");
+    return fail(DWFGPU_HIP_ERROR);
+  }
   if (hipHostMalloc((void**)&h->hscal, 8 * sizeof(double), hipHostMallocDefault) != hipSuccess) {
     set_err("dwfgpu_create: hipHostMalloc failed");
     return fail(DWFGPU_HIP_ERROR);

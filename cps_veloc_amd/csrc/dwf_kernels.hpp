@@ -1414,9 +1414,11 @@ k_cg_xp(Fld<PX> x, Fld<P> p, CFld<P> r, double a, double b, Geom g) {
 // pn = r + b p (the next CG direction into another buffer)
 template <class P>
 __global__ void __launch_bounds__(BLAS_THREADS)
-k_pnext(Fld<P> pn, CFld<P> r, CFld<P> p, double b, Geom g) {
+k_pnext(Fld<P> pn, CFld<P> r, CFld<P> p, const double* __restrict__ scal, Geom g) {
   typedef typename P::R R;
-  const R br = (R)b;
+  // beta = |r|^2 / the previous |r|^2 from the device scalars (k_reduce_cg_step)
+  const double rr = scal[0], ro = scal[2];
+  const R br = (R)(ro != 0.0 ? rr / ro : 0.0);
   DWFK_SITE_LOOP(k, g) {
     cx<R> rv[12], pv[12];
     ld_site<P>(rv, r, g, k);
@@ -1434,7 +1436,8 @@ template <class P>
 struct XFlush {
   const typename P::C* p[XFLUSH_MAX];
   const float* sc[XFLUSH_MAX];
-  double a[XFLUSH_MAX];
+  const double* alpha;  // device alphas of the steps (k_reduce_cg_step)
+  int ai[XFLUSH_MAX];   // index into alpha of each pending step
   int cnt;
 };
 template <class P, class PX>
@@ -1448,7 +1451,7 @@ k_x_flush(Fld<PX> x, XFlush<P> f, Geom g) {
     for (int j = 0; j < f.cnt; ++j) {
       cx<R> pv[12];
       ld_site<P>(pv, CFld<P>{f.p[j], f.sc[j]}, g, k);
-      const double a = f.a[j];
+      const double a = f.alpha[f.ai[j]];
 #pragma unroll
       for (int c = 0; c < 12; ++c)
         xv[c] = cx<RX>{(RX)(a * pv[c].re) + xv[c].re, (RX)(a * pv[c].im) + xv[c].im};
@@ -1549,6 +1552,32 @@ k_reduce_partials(const double* __restrict__ partial, int n, double* __restrict_
   for (; k < n; k += 1024) a0 += partial[k];
   const double t = block_sum((a0 + a1) + (a2 + a3), sh);
   if (threadIdx.x == 0) out[0] = t;
+}
+
+// End of one step of the sloppy CG: |r|^2 of the partials into scal[0],
+// its previous value into scal[2] (for beta), and the step's alpha =
+// |r|^2 / <p, A p> (the value cg_alpha gave the epilogue) into *alpha_out,
+// all on the device so that the host need not wait for the step.
+__global__ void __launch_bounds__(1024)
+k_reduce_cg_step(const double* __restrict__ partial, int n, double* __restrict__ scal,
+                 double* __restrict__ alpha_out) {
+  __shared__ double sh[16];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int k = threadIdx.x;
+  for (; k + 3 * 1024 < n; k += 4 * 1024) {
+    a0 += partial[k];
+    a1 += partial[k + 1024];
+    a2 += partial[k + 2 * 1024];
+    a3 += partial[k + 3 * 1024];
+  }
+  for (; k < n; k += 1024) a0 += partial[k];
+  const double t = block_sum((a0 + a1) + (a2 + a3), sh);
+  if (threadIdx.x == 0) {
+    const double rr = scal[0];
+    *alpha_out = cg_alpha(scal);
+    scal[2] = rr;
+    scal[0] = t;
+  }
 }
 
 // CPS half vector [s][cb site][24 reals] (FP64, host order) <-> device field
