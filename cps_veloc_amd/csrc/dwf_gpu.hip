@@ -766,8 +766,10 @@ int invcg_mixed(dwfgpu_s* h, const dwfgpu_cg_arg* arg, double delta, const DVec&
   // k+2 (alpha, beta and the x_s updates take their scalars on the device),
   // so the GPU never waits for the host between steps.  A reliable update is
   // therefore taken LAG steps after its condition first holds, with the
-  // state of the current step.
-  constexpr int LAG = 2, NEV = 4;
+  // state of the current step.  BF16 keeps the test of every step (LAG 0):
+  // its recurrence loses conjugacy between updates, and two-step-late
+  // updates more than doubled its iteration count at 24^3x64 (3178 -> 7019).
+  constexpr int LAG = std::is_same<PS, PBF16>::value ? 0 : 2, NEV = 4;
   struct Events {
     hipEvent_t e[NEV] = {};
     ~Events() {

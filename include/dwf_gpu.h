@@ -124,9 +124,10 @@ int dwfgpu_invcg_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol,
  * spinor precisions and reliable_delta, quda/wilson_quda.C:84,134-137).
  * A reliable update (sol += sloppy correction, r = src - A sol in FP64)
  * happens when the sloppy residual norm drops below delta (0 < delta < 1)
- * times the norm at the previous update, or meets the stop rule.  The host
- * tests each sloppy step's residual two steps later (the GPU runs ahead),
- * so updates and the stop come two steps after their condition holds.
+ * times the norm at the previous update, or meets the stop rule.  For
+ * FP32 and I16 the host tests each sloppy step's residual two steps later
+ * (the GPU runs ahead), so updates and the stop come two steps after their
+ * condition holds; BF16 tests every step as it ends.
  * sloppy = DWFGPU_F64 runs the plain CG of dwfgpu_invcg (per-step test).
  * The handle's API precision must be DWFGPU_F64.
  * *iters: sloppy iterations + 1; *reliable_updates: their count.           */

@@ -52,6 +52,8 @@ CASES = [
     ("mobius", (4, 4, 4, 4), 4),
     ("mobius", (4, 2, 6, 4), 6),
     ("mobius", (8, 8, 4, 8), 12),
+    ("mobius", (4, 4, 4, 4), 8),   # the M5^-1 column swept in registers: Ls 8,
+    ("mobius", (4, 4, 2, 4), 16),  # 12 and 16 (other Ls take the LDS loop)
 ]
 
 
