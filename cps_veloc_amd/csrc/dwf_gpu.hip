@@ -602,19 +602,29 @@ struct Ops {
     int max_itr = arg->max_num_iter - 1;
     if (rr <= stp_cnd) max_itr = 0;
     int status = DWFGPU_OK;
-    // dscal[0] holds rr (fetch_sum above); each step replaces it by the new rr
+    // dscal[0] holds rr (fetch_sum above); each step replaces it by the new
+    // rr.  The hop epilogue updates res only; sol += a dir and dir = res + b
+    // dir follow in one pass (k_cg_xp), which reads dir once for both
     for (int k = 0; k < max_itr; ++k) {
       itr = itr + 1;
       const double rr_prv = rr;
       double d;
-      TRY(mdagm_cg<P>(h, mass, dir, res, sol, &rr, &d));
-      if (d == 0.0) {  // a = 0 was used: sol and res are unchanged
+      TRY(mdagm_cg<dwfk::NoX>(h, mass, dir, res, DVec(), &rr, &d));
+      if (d == 0.0) {  // a = 0 was used: res is unchanged
         status = DWFGPU_BREAKDOWN;
         break;
       }
-      if (rr <= stp_cnd) break;
+      const double a = rr_prv / d;  // the epilogue's alpha (cg_alpha)
+      if (rr <= stp_cnd) {
+        hipLaunchKernelGGL((dwfk::k_cg_xp<P, P, false>), blas_grid(), blas_block(), 0, h->st,
+                           fld<P>(sol), fld<P>(dir), cfld<P>(res), a, 0.0, h->g);
+        HIPCHK(hipGetLastError());
+        break;
+      }
       const double b = rr / rr_prv;
-      TRY(axpby(h, dir, res, 1.0, b));  // dir = res + b dir
+      hipLaunchKernelGGL((dwfk::k_cg_xp<P, P, true>), blas_grid(), blas_block(), 0, h->st,
+                         fld<P>(sol), fld<P>(dir), cfld<P>(res), a, b, h->g);
+      HIPCHK(hipGetLastError());
     }
     // true residual |src - A sol| / |src|  (inv_cg.C:403-415)
     TRY(mdagm(h, mass, mmp, sol, nullptr));
