@@ -606,9 +606,10 @@ __device__ __forceinline__ void link_site(const Geom& g, const int* c, int i, bo
 // (d < 4) or x - (d-4) that the hop reads, so no slice recomputes it.
 // one entry of the neighbour table (layout: Geom::nbt)
 __global__ void __launch_bounds__(256) k_build_nbt(int* __restrict__ nbt, Geom g) {
-  const int n = 2 * 8 * g.vh;
-  for (int w = blockIdx.x * blockDim.x + threadIdx.x; w < n; w += gridDim.x * blockDim.x) {
-    const int i = w % g.vh, d = (w / g.vh) % 8, p = w / (8 * g.vh);
+  const size_t n = (size_t)2 * 8 * g.vh;
+  for (size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x; w < n;
+       w += (size_t)gridDim.x * blockDim.x) {
+    const int i = (int)(w % g.vh), d = (int)((w / g.vh) % 8), p = (int)(w / (8 * (size_t)g.vh));
     int c[4];
     site_coords(g, i, p, c);
     int site = i, nbr = i;
