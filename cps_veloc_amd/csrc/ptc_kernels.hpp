@@ -34,7 +34,11 @@ constexpr int NDIR = 9;
 constexpr int NT = 32;          // sites per tile
 constexpr int NCOL = 3 * NT;    // GEMM columns per tile
 constexpr int NTHR = 256;       // threads (4 waves) per work-group
-constexpr int PADK = 8;         // LDS row padding (bf16 elements)
+constexpr int PADK = 8;         // LDS row padding of [col][K'] tiles (bf16 elements)
+// row stride of the [K'][col] / [M'][col] tiles of the weight gradient: 100
+// bf16 = 50 dwords, so the consecutive rows a wavefront writes while building
+// a tile land in different LDS banks (a stride of 104 repeats every 16 rows)
+constexpr int ROWC = NCOL + 4;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -160,31 +164,64 @@ __device__ __forceinline__ size_t nbr_site(const int* lnb, int d, int n, int par
 // either [col][K'] (KMAJOR = false, the B operand of the forward GEMM) or
 // [K'][col] (KMAJOR = true, the Y operand of the weight gradient).
 // k' = part*(CIN*4) + i*4 + spin, col = n*3 + colour.
+// Staging loops issue their loads CHUNK items at a time before using any
+// (a few round trips to memory per tile instead of one per item, without
+// holding a whole tile in registers next to the MFMA accumulators).
+constexpr int CHUNK = 4;
+// the largest chunk of at most `cap` items that divides `per` (items per thread)
+constexpr int chunk_for(int per, int cap) {
+  int c = cap;
+  while (c > 1 && per % c != 0) --c;
+  return c;
+}
+
 template <int CIN, bool KMAJOR>
 __device__ __forceinline__ void build_transported(uint16_t* sB, const uint32_t* __restrict__ in,
                                                   const cx<float>* lu, const int* lnb, int d,
                                                   int par, int i0, const Geom& g) {
   constexpr int KP = 8 * CIN;
-  for (int w = threadIdx.x; w < NT * CIN * 4; w += NTHR) {
-    const int n = w / (CIN * 4), r = w - n * (CIN * 4);
-    const int i = r >> 2, sp = r & 3;
-    const size_t y = nbr_site(lnb, d, n, par, i0, g);
-    const uint32_t* src = in + (y * CIN + i) * 12 + sp * 3;
-    cx<float> f[3], v[3];
-    f[0] = unpack_c(src[0]);
-    f[1] = unpack_c(src[1]);
-    f[2] = unpack_c(src[2]);
-    transport3(d == 0 ? nullptr : lu + ((d - 1) * NT + n) * 9, d, f, v);
-    const int kre = i * 4 + sp, kim = CIN * 4 + i * 4 + sp;
+  constexpr int NITEM = NT * CIN * 4, PERT = (NITEM + NTHR - 1) / NTHR;
+  constexpr int CH = chunk_for(PERT, CHUNK);
+  // no item guards when the items split evenly into whole chunks
+  constexpr bool EXACT = NITEM % NTHR == 0 && PERT % CH == 0;
+#pragma unroll 1
+  for (int j0 = 0; j0 < PERT; j0 += CH) {
+    uint32_t raw[CH][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int col = n * 3 + c;
-      if (KMAJOR) {
-        sB[kre * (NCOL + PADK) + col] = f2bf(v[c].re);
-        sB[kim * (NCOL + PADK) + col] = f2bf(v[c].im);
-      } else {
-        sB[col * (KP + PADK) + kre] = f2bf(v[c].re);
-        sB[col * (KP + PADK) + kim] = f2bf(v[c].im);
+    for (int jj = 0; jj < CH; ++jj) {
+      const int w = threadIdx.x + (j0 + jj) * NTHR;
+      if (EXACT || (j0 + jj < PERT && w < NITEM)) {
+        const int n = w / (CIN * 4), r = w - n * (CIN * 4);
+        const int i = r >> 2, sp = r & 3;
+        const size_t y = nbr_site(lnb, d, n, par, i0, g);
+        const uint32_t* src = in + (y * CIN + i) * 12 + sp * 3;
+        raw[jj][0] = src[0];
+        raw[jj][1] = src[1];
+        raw[jj][2] = src[2];
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < CH; ++jj) {
+      const int w = threadIdx.x + (j0 + jj) * NTHR;
+      if (!EXACT && (j0 + jj >= PERT || w >= NITEM)) continue;
+      const int n = w / (CIN * 4), r = w - n * (CIN * 4);
+      const int i = r >> 2, sp = r & 3;
+      cx<float> f[3], v[3];
+      f[0] = unpack_c(raw[jj][0]);
+      f[1] = unpack_c(raw[jj][1]);
+      f[2] = unpack_c(raw[jj][2]);
+      transport3(d == 0 ? nullptr : lu + ((d - 1) * NT + n) * 9, d, f, v);
+      const int kre = i * 4 + sp, kim = CIN * 4 + i * 4 + sp;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const int col = n * 3 + c;
+        if (KMAJOR) {
+          sB[kre * ROWC + col] = f2bf(v[c].re);
+          sB[kim * ROWC + col] = f2bf(v[c].im);
+        } else {
+          sB[col * (KP + PADK) + kre] = f2bf(v[c].re);
+          sB[col * (KP + PADK) + kim] = f2bf(v[c].im);
+        }
       }
     }
   }
@@ -241,10 +278,27 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
   for (int d = 0; d < NDIR; ++d) {
     __syncthreads();  // links staged / previous direction's operands consumed
     {
+      // the direction's weights: every load issued before the stores
+      constexpr int NA = MP * KP / 8, PA = (NA + NTHR - 1) / NTHR;
+      constexpr int CH = chunk_for(PA, CHUNK);
+      constexpr bool EXACT = NA % NTHR == 0 && PA % CH == 0;
       const uint4* src = reinterpret_cast<const uint4*>(A + (size_t)d * MP * KP);
-      for (int w = threadIdx.x; w < MP * KP / 8; w += NTHR) {
-        const int m = w / (KP / 8), k8 = w - m * (KP / 8);
-        *reinterpret_cast<uint4*>(sA + m * (KP + PADK) + k8 * 8) = src[w];
+#pragma unroll 1
+      for (int j0 = 0; j0 < PA; j0 += CH) {
+        uint4 wv[CH];
+#pragma unroll
+        for (int jj = 0; jj < CH; ++jj) {
+          const int w = threadIdx.x + (j0 + jj) * NTHR;
+          if (EXACT || (j0 + jj < PA && w < NA)) wv[jj] = src[w];
+        }
+#pragma unroll
+        for (int jj = 0; jj < CH; ++jj) {
+          const int w = threadIdx.x + (j0 + jj) * NTHR;
+          if (EXACT || (j0 + jj < PA && w < NA)) {
+            const int m = w / (KP / 8), k8 = w - m * (KP / 8);
+            *reinterpret_cast<uint4*>(sA + m * (KP + PADK) + k8 * 8) = wv[jj];
+          }
+        }
       }
     }
     build_transported<CIN, false>(sB, in, lu, lnb, d, par, i0, g);
@@ -288,8 +342,8 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
 
 template <int CIN, int COUT>
 constexpr size_t gw_lds() {
-  constexpr size_t x = (size_t)(8 * COUT) * (NCOL + PADK) * 2;
-  constexpr size_t y = (size_t)(8 * CIN) * (NCOL + PADK) * 2;
+  constexpr size_t x = (size_t)(8 * COUT) * ROWC * 2;
+  constexpr size_t y = (size_t)(8 * CIN) * ROWC * 2;
   return x + y + (size_t)8 * NT * 9 * 8 + 8 * NT * 4;
 }
 
@@ -305,8 +359,8 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
   constexpr int NBLK = MB * KB2, PER = (NBLK + 3) / 4;
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t* sX = reinterpret_cast<uint16_t*>(smem);
-  uint16_t* sY = sX + MP * (NCOL + PADK);
-  cx<float>* lu = reinterpret_cast<cx<float>*>(smem + (size_t)(MP + KP) * (NCOL + PADK) * 2);
+  uint16_t* sY = sX + MP * ROWC;
+  cx<float>* lu = reinterpret_cast<cx<float>*>(smem + (size_t)(MP + KP) * ROWC * 2);
   int* lnb = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(lu) + (size_t)8 * NT * 9 * 8);
   const int d = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -319,13 +373,33 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
     const int i0 = (t - par * tiles_per_par) * NT;
     __syncthreads();  // previous tile's operands consumed
     dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par);
-    // X: the output gradient of the tile, [M'][col]
-    for (int w = threadIdx.x; w < NT * COUT * 12; w += NTHR) {
-      const int n = w / (COUT * 12), r = w - n * (COUT * 12);
-      const int os = r / 3, c = r - os * 3;
-      const cx<float> v = unpack_c(gout[((size_t)par * g.vh + i0 + n) * (COUT * 12) + r]);
-      sX[os * (NCOL + PADK) + n * 3 + c] = f2bf(v.re);
-      sX[(COUT * 4 + os) * (NCOL + PADK) + n * 3 + c] = f2bf(v.im);
+    // X: the output gradient of the tile, [M'][col]; the tile is contiguous
+    // in gout, and every load is issued before the stores
+    {
+      constexpr int NX = NT * COUT * 12, PX = (NX + NTHR - 1) / NTHR;
+      constexpr int CH = chunk_for(PX, 2 * CHUNK);
+      constexpr bool EXACT = NX % NTHR == 0 && PX % CH == 0;
+      const uint32_t* src = gout + ((size_t)par * g.vh + i0) * (COUT * 12);
+#pragma unroll 1
+      for (int j0 = 0; j0 < PX; j0 += CH) {
+        uint32_t xv[CH];
+#pragma unroll
+        for (int jj = 0; jj < CH; ++jj) {
+          const int w = threadIdx.x + (j0 + jj) * NTHR;
+          if (EXACT || (j0 + jj < PX && w < NX)) xv[jj] = src[w];
+        }
+#pragma unroll
+        for (int jj = 0; jj < CH; ++jj) {
+          const int w = threadIdx.x + (j0 + jj) * NTHR;
+          if (EXACT || (j0 + jj < PX && w < NX)) {
+            const int n = w / (COUT * 12), r = w - n * (COUT * 12);
+            const int os = r / 3, c = r - os * 3;
+            const cx<float> v = unpack_c(xv[jj]);
+            sX[os * ROWC + n * 3 + c] = f2bf(v.re);
+            sX[(COUT * 4 + os) * ROWC + n * 3 + c] = f2bf(v.im);
+          }
+        }
+      }
     }
     __syncthreads();  // links staged
     build_transported<CIN, true>(sY, in, lu, lnb, d, par, i0, g);
@@ -338,8 +412,8 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
         const int blk = wave + 4 * b;
         if (blk < NBLK) {
           const int mb = blk / KB2, kb = blk - mb * KB2;
-          const bf16x4 xf = ld4(sX + (mb * 16 + (lane & 15)) * (NCOL + PADK) + c0);
-          const bf16x4 yf = ld4(sY + (kb * 16 + (lane & 15)) * (NCOL + PADK) + c0);
+          const bf16x4 xf = ld4(sX + (mb * 16 + (lane & 15)) * ROWC + c0);
+          const bf16x4 yf = ld4(sY + (kb * 16 + (lane & 15)) * ROWC + c0);
           acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xf, yf, acc[b], 0, 0, 0);
         }
       }
