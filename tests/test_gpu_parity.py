@@ -54,6 +54,8 @@ CASES = [
     ("mobius", (8, 8, 4, 8), 12),
     ("mobius", (4, 4, 4, 4), 8),   # the M5^-1 column swept in registers: Ls 8,
     ("mobius", (4, 4, 2, 4), 16),  # 12 and 16 (other Ls take the LDS loop)
+    ("dwf", (6, 2, 2, 2), 4),      # 24 cb sites: the second work-group is half
+    ("mobius", (6, 2, 2, 2), 4),   # empty (inactive lanes), odd Lx/2
 ]
 
 
@@ -82,7 +84,7 @@ def test_matpc(action, dims, ls, dag):
     assert site_rel(got, want) < 1e-13
 
 
-@pytest.mark.parametrize("action,dims,ls", CASES[:2] + CASES[3:5])
+@pytest.mark.parametrize("action,dims,ls", CASES[:2] + CASES[3:5] + CASES[-2:])
 @pytest.mark.parametrize("cb", [0, 1])
 @pytest.mark.parametrize("dag", [0, 1])
 def test_dslash(action, dims, ls, cb, dag):
@@ -99,6 +101,7 @@ def test_dslash(action, dims, ls, cb, dag):
     ("dwf", (8, 8, 8, 8), 8, 0.04),        # 8^3x8, Ls=8, M5=1.8, mf=0.04
     ("mobius", (4, 4, 4, 8), 8, 0.04),
     ("mobius", (4, 2, 6, 4), 6, 0.1),
+    ("dwf", (6, 2, 2, 2), 4, 0.1),
 ])
 def test_invcg_matches_oracle(action, dims, ls, mass):
     say(f"[gpu] InvCg {action} {dims} ls={ls} mf={mass}")
