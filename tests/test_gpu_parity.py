@@ -162,6 +162,25 @@ def test_fp32_storage(action):
 
 
 @pytest.mark.parametrize("action", ["dwf", "mobius"])
+def test_invcg_fp32_handle(action):
+    """InvCg on a handle with FP32 storage: the same CG (r-only hop epilogue,
+    then x and p in one pass) converges to a stop target FP32 can reach, in
+    about the oracle's FP64 iteration count for that target."""
+    say(f"[gpu] InvCg FP32 handle {action}")
+    dims, ls, mass = (4, 4, 4, 8), 8, 0.04
+    U, o, g = make(action, dims, ls, precision="f32")
+    src = src_vec(dims, ls, 31)
+    osol, oit, otr, _ = o.invcg(src, mass, stop_rsd=1e-5, max_iter=2000)
+    gsol = np.zeros(g.n)
+    git, gtr, st = g.InvCg(gsol, src, CgArg(mass, 2000, 1e-5))
+    say(f"  oracle iters={oit}  gpu iters={git} true_res={gtr:.3e}")
+    assert st == 0
+    assert abs(git - oit) <= max(2, oit // 10)
+    assert gtr < 2e-5
+    assert np.linalg.norm(gsol - osol) / np.linalg.norm(osol) < 1e-3
+
+
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
 def test_device_vectors(action):
     say(f"[gpu] device vectors {action}")
     dims, ls = (4, 4, 4, 4), 4
