@@ -388,7 +388,12 @@ struct Ops {
     const MobArgs a = mob_args(h, mass, dag);
     const S* u = (const S*)F(h).U;
     const bool col = dag || epi == dwfk::EPI_M5INV || epi == dwfk::EPI_SCHUR;
-    const size_t split_lds = dwfk::split_col_offset<R>(h->ns) + col_lds_bytes(h, sizeof(R));
+    // exchange slots after the links; the M5^-1 / Schur s-columns reuse the
+    // LDS from offset 0 once the hop is done
+    const size_t split_lds =
+        std::max(col ? col_lds_bytes(h, sizeof(R)) : (size_t)0,
+                 dwfk::split_col_offset<R>(h->ns) +
+                     (size_t)h->ls * h->ns * dwfk::TX_SLOT * 2 * sizeof(R));
     const bool split = dwfk::MobSplitT<P>::value && !dag && !(fl & dwfk::FL_PRE_T) &&
                        split_lds <= (size_t)h->max_lds;
     const size_t lds = split ? split_lds

@@ -698,10 +698,8 @@ __device__ __forceinline__ void hop_dir(cx<R>* acc, const LD& ld, const cx<R>* l
   }
 }
 
-// The hop of one chirality of its input: acc += hop(P+ psi) (UP) or
-// hop(P- psi).  The projection of P+ psi is its upper two spins as they are,
-// that of P- psi the sign/phase-mapped lower two, so each pass reads half of
-// every neighbour spinor (the unused chunk loads vanish).
+// acc += the reconstructed (1 -/+ gamma_MU) U h of one projected neighbour:
+// U h (FWD) or U^dag h, scaled by sc
 template <int MU, int SG, bool FWD, typename R>
 __device__ __forceinline__ void hop_half(cx<R>* acc, const cx<R>* h, const cx<R>* l, R sc) {
   cx<R> r[6];
@@ -716,45 +714,68 @@ __device__ __forceinline__ void hop_half(cx<R>* acc, const cx<R>* h, const cx<R>
   reconstruct<MU, SG>(acc, r, sc);
 }
 
-template <int MU, int SG, bool UP, typename R, class LD>
-__device__ __forceinline__ void chiral_proj(const LD& ld, int n, cx<R>* h, R* sc) {
-  cx<R> p[12];
-  *sc = (R)ld.load(n, p);
-  if (UP) {
+// keeps the accumulated spinor computed where it stands: without it the
+// compiler sinks the arithmetic of the exchange gather below into the
+// (act) store after its last barrier and keeps every loaded operand alive
+// until then
+template <class R>
+__device__ __forceinline__ void pin12(cx<R>* v) {
 #pragma unroll
-    for (int c = 0; c < 6; ++c) h[c] = p[c];
-  } else {
-#pragma unroll
-    for (int c = 0; c < 6; ++c) p[c] = cx<R>{0, 0};
-    project<MU, SG>(p, h);
-  }
+  for (int c = 0; c < 12; ++c) asm volatile("" : "+v"(v[c].re), "+v"(v[c].im));
 }
 
-template <int MU, int DAG, bool UP, typename R, class LD>
-__device__ __forceinline__ void hop_dir_chiral(cx<R>* acc, const LD& ld, const cx<R>* lf,
-                                               const cx<R>* lb, const Nbr& nb) {
-  constexpr int SF = DAG ? 1 : -1;
-  cx<R> h[6];
-  R sc;
-  chiral_proj<MU, SF, UP>(ld, nb.fwd[MU], h, &sc);
-  hop_half<MU, SF, true>(acc, h, lf, sc);
-  chiral_proj<MU, -SF, UP>(ld, nb.bwd[MU], h, &sc);
-  hop_half<MU, -SF, false>(acc, h, lb, sc);
+// One direction of hop((b + c D5) in) at slice s, dag 0, reading every
+// neighbour spinor once.  (b + c D5) in (s) = b in(s) + c P+ in(s-1) +
+// c P- in(s+1) (zmobius_dslash_4.C:90-105), P+ in being the upper two spins
+// in the chiral basis.  Each thread loads slice s of the neighbour; the upper
+// half of slice s-1 and the lower half of slice s+1 come from the threads of
+// those slices through the exchange slots `mine` / `prv` / `nxt` (6 complex
+// a thread).  Four barriers: every thread of the work-group must call it.
+template <int MU, int SG, bool FWD, typename R, class LD>
+__device__ __forceinline__ void hop_dir_tx(cx<R>* acc, LD ld, int n, const cx<R>* l,
+                                           cx<R>* mine, const cx<R>* prv, const cx<R>* nxt,
+                                           R b, R cm, R cp) {
+  cx<R> p[12], q[6], h[6];
+  ld.load(n, p);  // unscaled storage only (MobSplitT)
+  __syncthreads();  // the previous direction's readers are done with the slots
+#pragma unroll
+  for (int c = 0; c < 6; ++c) mine[c] = p[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 6; ++c) q[c] = prv[c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 6; ++c) mine[c] = p[6 + c];
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < 6; ++c) p[c] = b * p[c] + cm * q[c];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) p[6 + c] = b * p[6 + c] + cp * nxt[c];
+  project<MU, SG>(p, h);
+  hop_half<MU, SG, FWD>(acc, h, l, (R)1);
+  pin12(acc);
 }
 
-template <int DAG, bool UP, typename R, class LD>
-__device__ __forceinline__ void hop4_chiral(cx<R>* acc, const LD& ld, const cx<R>* lu, int ds,
-                                            const int* nbi, int ns) {
+// acc = hop((b + c D5) in) at slice s (dag 0) through hop_dir_tx; inactive
+// lanes read site 0 and take the barriers with the others
+template <typename R, class LD>
+__device__ __forceinline__ void hop4_tx(cx<R>* acc, LD ld, const cx<R>* lu, int ds,
+                                        const int* nbi, int ns, bool act, cx<R>* mine,
+                                        const cx<R>* prv, const cx<R>* nxt, R b, R cm, R cp) {
   Nbr nb;
 #pragma unroll
   for (int mu = 0; mu < 4; ++mu) {
-    nb.fwd[mu] = nbi[mu * ns];
-    nb.bwd[mu] = nbi[(4 + mu) * ns];
+    nb.fwd[mu] = act ? nbi[mu * ns] : 0;
+    nb.bwd[mu] = act ? nbi[(4 + mu) * ns] : 0;
   }
-  hop_dir_chiral<0, DAG, UP>(acc, ld, lu + 0 * ds, lu + 4 * ds, nb);
-  hop_dir_chiral<1, DAG, UP>(acc, ld, lu + 1 * ds, lu + 5 * ds, nb);
-  hop_dir_chiral<2, DAG, UP>(acc, ld, lu + 2 * ds, lu + 6 * ds, nb);
-  hop_dir_chiral<3, DAG, UP>(acc, ld, lu + 3 * ds, lu + 7 * ds, nb);
+  hop_dir_tx<0, -1, true>(acc, ld, nb.fwd[0], lu + 0 * ds, mine, prv, nxt, b, cm, cp);
+  hop_dir_tx<0, 1, false>(acc, ld, nb.bwd[0], lu + 4 * ds, mine, prv, nxt, b, cm, cp);
+  hop_dir_tx<1, -1, true>(acc, ld, nb.fwd[1], lu + 1 * ds, mine, prv, nxt, b, cm, cp);
+  hop_dir_tx<1, 1, false>(acc, ld, nb.bwd[1], lu + 5 * ds, mine, prv, nxt, b, cm, cp);
+  hop_dir_tx<2, -1, true>(acc, ld, nb.fwd[2], lu + 2 * ds, mine, prv, nxt, b, cm, cp);
+  hop_dir_tx<2, 1, false>(acc, ld, nb.bwd[2], lu + 6 * ds, mine, prv, nxt, b, cm, cp);
+  hop_dir_tx<3, -1, true>(acc, ld, nb.fwd[3], lu + 3 * ds, mine, prv, nxt, b, cm, cp);
+  hop_dir_tx<3, 1, false>(acc, ld, nb.bwd[3], lu + 7 * ds, mine, prv, nxt, b, cm, cp);
 }
 
 // acc = D_hop^(dag) psi at output site i of parity p_out (wilson_dslash.C:173-517);
@@ -1045,8 +1066,9 @@ enum { FL_NORM = 1, FL_PRE_T = 2, FL_POST_T = 4, FL_CG = 8, FL_SPLIT_T = 16 };
 // FL_POST_T : dag 0 EPI_M5INV, write (b + c D5) M5^-1 (...) so that the next
 //             pass can use FL_PRE_T
 // FL_CG     : dag 1 EPI_SCHUR, feed the result to cg_epilogue instead of `out`
-// FL_SPLIT_T: dag 0 without FL_PRE_T, gather (b + c D5) in by chirality
-//             (two half-spinor passes, LDS after the links: split_col_offset)
+// FL_SPLIT_T: dag 0 without FL_PRE_T, gather (b + c D5) in reading each
+//             neighbour once (hop4_tx; exchange slots after the links:
+//             split_col_offset)
 template <typename R>
 __device__ __forceinline__ void apply_t_col(cx<R>* tv, const cx<R>* col_site, int s, int ls,
                                             const MobArgs& a, int dag) {
@@ -1082,16 +1104,18 @@ __device__ __forceinline__ void m5_term_col(cx<R>* v, const cx<R>* col_site, int
   }
 }
 
-// whether the dag-0 gather of (b + c D5) in is split by chirality (each
-// neighbour read once, twice the SU(3) products) rather than reading the
-// slices s-1, s, s+1 of every neighbour: FP64, where the kernel waits on
-// memory rather than on arithmetic
+// whether the dag-0 gather of (b + c D5) in reads every neighbour once and
+// trades the halves of slices s-1 / s+1 through LDS (hop4_tx) rather than
+// reading the slices s-1, s, s+1 of every neighbour: FP64, whose spinors are
+// stored unscaled and whose kernel waits on memory
 template <class P>
 struct MobSplitT {
   static constexpr bool value = std::is_same<P, P64>::value;
 };
-// byte offset of the s-column of the split gather: after the staged
-// neighbour indices and links, 16-byte aligned
+// byte offset of the exchange slots of hop4_tx ([ls][ns] slots of
+// TX_SLOT complex): after the staged neighbour indices and links, 16-byte
+// aligned
+constexpr int TX_SLOT = 7;  // 6 complex, padded against LDS bank conflicts
 template <typename R>
 __host__ __device__ constexpr size_t split_col_offset(int ns) {
   return ((size_t)8 * ns * 4 + (size_t)8 * ns * 9 * 2 * sizeof(R) + 15) / 16 * 16;
@@ -1131,34 +1155,14 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
   constexpr bool split_t = DAG == 0 && !(FL & FL_PRE_T) && (FL & FL_SPLIT_T);
   cx<R>* mycol = col + il * sst;
   if constexpr (split_t) {
-    // hop((b + c D5) in)(s) = b (A + B)(s) + c A(s-1) + c B(s+1) with
-    // A = hop(P+ in), B = hop(P- in) (the hop acts on every slice alike).
-    // Two passes over the directions, each reading one half of every
-    // neighbour spinor: the neighbours are read once in all, instead of
-    // the slices s-1, s, s+1 of each.  A waits in an LDS s-column placed
-    // after the staged links (the links serve both passes).
     const PlainLoader<P> ld{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh};
-    cx<R>* cola = reinterpret_cast<cx<R>*>(smem_raw + split_col_offset<R>(ns)) + il * sst;
-    if (act) hop4_chiral<0, true>(acc, ld, lu + il * 9, ns * 9, lnb + il, ns);
-#pragma unroll
-    for (int c = 0; c < 12; ++c) cola[s * 12 + c] = acc[c];
-    cx<R> accb[12];
-#pragma unroll
-    for (int c = 0; c < 12; ++c) accb[c] = cx<R>{0, 0};
-    if (act) hop4_chiral<0, false>(accb, ld, lu + il * 9, ns * 9, lnb + il, ns);
+    cx<R>* tx = reinterpret_cast<cx<R>*>(smem_raw + split_col_offset<R>(ns));
     const int sm = s > 0 ? s - 1 : ls - 1, sp = s < ls - 1 ? s + 1 : 0;
     const R b = pick<R>(a.b, a.fb);
     const R cm = s > 0 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
     const R cp = s < ls - 1 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
-    __syncthreads();  // A of every slice is in LDS
-#pragma unroll
-    for (int c = 0; c < 12; ++c) acc[c] = b * (acc[c] + accb[c]) + cm * cola[sm * 12 + c];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 12; ++c) cola[s * 12 + c] = accb[c];
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < 12; ++c) acc[c] = acc[c] + cp * cola[sp * 12 + c];
+    hop4_tx(acc, ld, lu + il * 9, ns * 9, lnb + il, ns, act, tx + (s * ns + il) * TX_SLOT,
+            tx + (sm * ns + il) * TX_SLOT, tx + (sp * ns + il) * TX_SLOT, b, cm, cp);
   } else if (act) {
     if (DAG == 0 && !(FL & FL_PRE_T)) {
       MobTLoader<P> ld;
