@@ -736,7 +736,9 @@ __device__ __forceinline__ void hop_dir_tx(cx<R>* acc, LD ld, int n, const cx<R>
                                            cx<R>* mine, const cx<R>* prv, const cx<R>* nxt,
                                            R b, R cm, R cp) {
   cx<R> p[12], q[6], h[6];
-  ld.load(n, p);  // unscaled storage only (MobSplitT)
+  const R sc = (R)ld.load(n, p);
+#pragma unroll
+  for (int c = 0; c < 12; ++c) p[c] = sc * p[c];
   __syncthreads();  // the previous direction's readers are done with the slots
 #pragma unroll
   for (int c = 0; c < 6; ++c) mine[c] = p[c];
@@ -1106,11 +1108,13 @@ __device__ __forceinline__ void m5_term_col(cx<R>* v, const cx<R>* col_site, int
 
 // whether the dag-0 gather of (b + c D5) in reads every neighbour once and
 // trades the halves of slices s-1 / s+1 through LDS (hop4_tx) rather than
-// reading the slices s-1, s, s+1 of every neighbour: FP64, whose spinors are
-// stored unscaled and whose kernel waits on memory
+// reading the slices s-1, s, s+1 of every neighbour: FP64 and FP32, whose
+// kernels wait on memory.  The 16-bit kernels are bound by instruction issue
+// and keep the three-slice gather, which projects before converting (the
+// exchange made them 7-8 % slower)
 template <class P>
 struct MobSplitT {
-  static constexpr bool value = std::is_same<P, P64>::value;
+  static constexpr bool value = std::is_same<P, P64>::value || std::is_same<P, P32>::value;
 };
 // byte offset of the exchange slots of hop4_tx ([ls][ns] slots of
 // TX_SLOT complex): after the staged neighbour indices and links, 16-byte
