@@ -54,6 +54,7 @@ CASES = [
     ("mobius", (8, 8, 4, 8), 12),
     ("mobius", (4, 4, 4, 4), 8),   # the M5^-1 column swept in registers: Ls 8,
     ("mobius", (4, 4, 2, 4), 16),  # 12 and 16 (other Ls take the LDS loop)
+    ("mobius", (4, 4, 4, 4), 2),   # Ls 2: slices s-1 and s+1 are one slice
     ("dwf", (6, 2, 2, 2), 4),      # 24 cb sites: the second work-group is half
     ("mobius", (6, 2, 2, 2), 4),   # empty (inactive lanes), odd Lx/2
 ]
@@ -148,10 +149,14 @@ def test_matinv_matches_oracle(action):
     assert np.linalg.norm(got - want) / np.linalg.norm(want) < 1e-9
 
 
-@pytest.mark.parametrize("action", ["dwf", "mobius"])
-def test_fp32_storage(action):
-    say(f"[gpu] FP32 storage {action}")
-    dims, ls = (4, 4, 4, 8), 8
+@pytest.mark.parametrize("action,dims,ls", [
+    ("dwf", (4, 4, 4, 8), 8),
+    ("mobius", (4, 4, 4, 8), 8),
+    ("mobius", (6, 2, 2, 2), 4),   # a half-empty work-group in the slice exchange
+    ("mobius", (4, 4, 4, 4), 2),
+])
+def test_fp32_storage(action, dims, ls):
+    say(f"[gpu] FP32 storage {action} {dims} ls={ls}")
     U, o, g = make(action, dims, ls, precision="f32")
     x = src_vec(dims, ls, 23)
     want, dwant = o.matpcdagmatpc(x, 0.04)
