@@ -300,6 +300,24 @@ __device__ __forceinline__ void ld_spinor_once(const typename P::C* __restrict__
   }
 }
 
+// one stored link element; non-temporal in FP64: a link is staged by two
+// work-groups (as U_mu(x) and as U_mu(x - mu)), mostly far apart in time,
+// and the spinor gather needs L2 more (FP64 MatPcDagMatPc 2 % faster; the
+// FP32 links of a 16^3x32 lattice stay in the last-level cache between
+// passes, and bypassing it made that operator 6 % slower)
+template <class P>
+__device__ __forceinline__ typename P::S ld_link_elem(const typename P::S* __restrict__ p) {
+  typename P::S x;
+  if constexpr (std::is_same<P, P64>::value) {
+    static_assert(sizeof(x) == 16, "FP64 link elements are 16 bytes");
+    const u32x4_t w = __builtin_nontemporal_load(reinterpret_cast<const u32x4_t*>(p));
+    __builtin_memcpy(&x, &w, 16);
+  } else {
+    x = *p;
+  }
+  return x;
+}
+
 // one 16-byte chunk of policy P stored non-temporally in FP64 / FP32 (see
 // ld_spinor_once): the outputs of a pass are not re-read by it, so they
 // should not evict its neighbours from L2
@@ -692,7 +710,7 @@ __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
     cx<R>* dst = lu + (size_t)w * 9;
     const float ls = link_scale<P>();
 #pragma unroll
-    for (int e = 0; e < NE; ++e) dst[e] = Pack<P>::unpack(src[(size_t)e * g.vh], ls);
+    for (int e = 0; e < NE; ++e) dst[e] = Pack<P>::unpack(ld_link_elem<P>(src + (size_t)e * g.vh), ls);
     if (NE == 6) {
       // row 2 = conj(row0 x row1) (det U = 1), times the boundary sign
       const cx<R> a0 = dst[0], a1 = dst[1], a2 = dst[2], b0 = dst[3], b1 = dst[4], b2 = dst[5];
