@@ -365,9 +365,11 @@ struct Ops {
     const double two_a5 = 2.0 * h->a5inv, nm = -2.0 * mass * h->a5inv;
     const S* u = (const S*)F(h).U;
     const size_t lds = link_lds_bytes(h, 2, sizeof(R));
+    Geom gm = h->g;
+    gm.rev = epi;  // the x0 + alpha D passes sweep backwards, as the Schur ones in mob()
 #define DWF_LAUNCH(DAG, EPI, NORM)                                                           \
   hipLaunchKernelGGL((dwfk::k_dwf_dslash<P, DAG, EPI, NORM>), grid, block, lds, h->st,       \
-                     fld<P>(out), cfld<P>(in), u, cfld<P>(x0), h->partial, h->g, cb_in, two_a5, \
+                     fld<P>(out), cfld<P>(in), u, cfld<P>(x0), h->partial, gm, cb_in, two_a5, \
                      nm, alpha, dwfk::CgEpi<P, P>{})
     if (epi == 0) {
       if (dag) DWF_LAUNCH(1, 0, false); else DWF_LAUNCH(0, 0, false);
@@ -386,6 +388,10 @@ struct Ops {
                  int p_out, int dag, int epi, int fl) {
     const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const MobArgs a = mob_args(h, mass, dag);
+    // the Schur passes sweep backwards: they start on the sites the pass
+    // before them wrote last, which the last-level cache still holds
+    Geom gm = h->g;
+    gm.rev = epi == dwfk::EPI_SCHUR ? 1 : 0;
     const S* u = (const S*)F(h).U;
     const bool col = dag || epi == dwfk::EPI_M5INV || epi == dwfk::EPI_SCHUR;
     // exchange slots after the links; the M5^-1 / Schur s-columns reuse the
@@ -401,7 +407,7 @@ struct Ops {
                                         col ? col_lds_bytes(h, sizeof(R)) : (size_t)0);
 #define MOB_LAUNCH(DAG, EPI, FL)                                                              \
   hipLaunchKernelGGL((dwfk::k_mobius<P, DAG, EPI, FL>), grid, block, lds, h->st, fld<P>(out), \
-                     cfld<P>(in), cfld<P>(psi), u, h->partial, h->g, p_out, a, dwfk::CgEpi<P, P>{})
+                     cfld<P>(in), cfld<P>(psi), u, h->partial, gm, p_out, a, dwfk::CgEpi<P, P>{})
     using dwfk::FL_NORM;
     using dwfk::FL_POST_T;
     using dwfk::FL_PRE_T;
@@ -509,9 +515,11 @@ struct Ops {
       TRY(dwf(h, mass, t1, te, 0, 0, 1, p, mk2, true));
       TRY(reduce_to(h, h->grid_hop, 1));
       TRY(dwf(h, mass, te, t1, 1, 1, 0, DVec(), 0.0, false));
+      Geom gm = h->g;
+      gm.rev = 1;  // an x0 + alpha D pass: backwards, as in dwf()
       hipLaunchKernelGGL((dwfk::k_dwf_dslash<P, 1, 1, false, true, PX>), grid, block,
                          link_lds_bytes(h, 2, sizeof(R)), h->st, fld<P>(te), cfld<P>(te), u,
-                         cfld<P>(t1), h->partial, h->g, 0, 2.0 * h->a5inv,
+                         cfld<P>(t1), h->partial, gm, 0, 2.0 * h->a5inv,
                          -2.0 * mass * h->a5inv, mk2, cg);
     } else {
       TRY(mob_first(h, mass, te, p, 0));
@@ -520,8 +528,10 @@ struct Ops {
       TRY(mob_first(h, mass, te, t1, 1));
       const MobArgs a = mob_args(h, mass, 1);
       const size_t lds = std::max(link_lds_bytes(h, 1, sizeof(R)), col_lds_bytes(h, sizeof(R)));
+      Geom gm = h->g;
+      gm.rev = 1;  // a Schur pass: backwards, as in mob()
       hipLaunchKernelGGL((dwfk::k_mobius<P, 1, dwfk::EPI_SCHUR, dwfk::FL_CG, PX>), grid, block,
-                         lds, h->st, fld<P>(te), cfld<P>(te), cfld<P>(t1), u, h->partial, h->g, 1,
+                         lds, h->st, fld<P>(te), cfld<P>(te), cfld<P>(t1), u, h->partial, gm, 1,
                          a, cg);
     }
     HIPCHK(hipGetLastError());

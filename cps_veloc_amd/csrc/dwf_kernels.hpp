@@ -365,6 +365,7 @@ struct Geom {
   // x - (d - 4), bit 31 set when the link between them carries the
   // anti-periodic sign (built once by k_build_nbt)
   const int* nbt;
+  int rev;  // 1: each XCD sweeps its range of work-groups backwards (xcd_block)
 };
 
 // site index of (x/2, y, z, t): order (fastest first) x/2, t%bt, z%bz, y,
@@ -433,11 +434,12 @@ __host__ __device__ __forceinline__ Nbr neighbours(const Geom& g, int i, int par
 // Work-groups are dispatched round-robin over the XCDs (hardware id b runs on
 // XCD b % nxcd).  Renumber them so that XCD k receives one contiguous range of
 // logical ids; neighbouring sites then share that XCD's L2.
-__device__ __forceinline__ int xcd_block(int nxcd) {
+__device__ __forceinline__ int xcd_block(int nxcd, int rev = 0) {
   const int b = blockIdx.x, nb = gridDim.x;
   const int q = nb / nxcd, r = nb - q * nxcd;
   const int k = b % nxcd, j = b / nxcd;
-  return k * q + (k < r ? k : r) + j;
+  const int cnt = q + (k < r ? 1 : 0);  // work-groups of XCD k; j < cnt
+  return k * q + (k < r ? k : r) + (rev ? cnt - 1 - j : j);
 }
 
 // ---------------------------------------------------------------------------
@@ -961,7 +963,7 @@ k_dwf_dslash(Fld<P> out, CFld<P> in, const typename P::S* __restrict__ U, CFld<P
   int* lnb = reinterpret_cast<int*>(smem_raw);                   // [2][8][ns]
   cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw + 2 * 8 * ns * 4);  // [2][8][ns][9]
   const double a_cg = CG ? cg_alpha(cg.scal) : 0.0;
-  const int blk = xcd_block(g.nxcd);
+  const int blk = xcd_block(g.nxcd, g.rev);
   const int i = blk * ns + il;
   // slice s holds output parity (s + 1 + cb_in) % 2: set s % 2 of the stage
   stage_links<P>(lu, lnb, U, g, blk, ns, 2, (1 + cb_in) & 1);
@@ -1203,7 +1205,7 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
   cx<R>* col = reinterpret_cast<cx<R>*>(smem_raw);
   const int sst = g.ls * 12 + 1;                    // padded site stride in LDS
   const double a_cg = (FL & FL_CG) ? cg_alpha(cg.scal) : 0.0;
-  const int blk = xcd_block(g.nxcd);
+  const int blk = xcd_block(g.nxcd, g.rev);
   const int i = blk * ns + il;
   const bool act = i < g.vh;
   const int nvalid = min(ns, g.vh - blk * ns);
@@ -1351,7 +1353,7 @@ k_m5inv(Fld<P> out, CFld<P> in, Geom g, M5Side k) {
   cx<R>* col = reinterpret_cast<cx<R>*>(smem_raw);
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;
   const int sst = g.ls * 12 + 1;
-  const int blk = xcd_block(g.nxcd);
+  const int blk = xcd_block(g.nxcd, g.rev);
   const int i = blk * ns + il;
   const bool act = i < g.vh;
   const int nvalid = min(ns, g.vh - blk * ns);
@@ -1974,7 +1976,7 @@ __global__ void __launch_bounds__(512) k_dwf_mrhs(MrArgs<P, PX> a) {
   int* lnb = reinterpret_cast<int*>(smem_raw);
   cx<R>* lu = reinterpret_cast<cx<R>*>(smem_raw + 2 * 8 * ns * 4);
   double* red = reinterpret_cast<double*>(smem_raw + 2 * 8 * ns * 4 + 2 * 8 * ns * 9 * sizeof(cx<R>));
-  const int blk = xcd_block(g.nxcd);
+  const int blk = xcd_block(g.nxcd, g.rev);
   const int i = blk * ns + il;
   stage_links<P>(lu, lnb, a.U, g, blk, ns, 2, (1 + a.cb_in) & 1);
   __syncthreads();
