@@ -55,6 +55,7 @@ CASES = [
     ("mobius", (4, 4, 4, 4), 8),   # the M5^-1 column swept in registers: Ls 8,
     ("mobius", (4, 4, 2, 4), 16),  # 12 and 16 (other Ls take the LDS loop)
     ("mobius", (4, 4, 4, 4), 2),   # Ls 2: slices s-1 and s+1 are one slice
+    ("mobius", (4, 4, 2, 4), 24),  # Ls 24: the s-columns need 8 sites per work-group
     ("dwf", (6, 2, 2, 2), 4),      # 24 cb sites: the second work-group is half
     ("mobius", (6, 2, 2, 2), 4),   # empty (inactive lanes), odd Lx/2
 ]
