@@ -597,18 +597,23 @@ struct Ops {
     return DWFGPU_OK;
   }
 
-  // DiracOp::InvCgShift, shift = NULL (inv_cg.C:84-448)
+  // DiracOp::InvCgShift(out, in, src_norm_sq, true_res, shift)
+  // (inv_cg.C:84-448).  src_norm_sq == 0: |src|^2 is computed (:166-169).
+  // shift != NULL: the solve is (MatPcDagMatPc + *shift) sol = src, with
+  // A p + shift p formed after the operator and d = <p, A p> + shift |p|^2
+  // (:248-251, 304-310), so its steps run unfused (invcg_shifted_step).
   static int invcg(dwfgpu_s* h, const dwfgpu_cg_arg* arg, const DVec& sol, const DVec& src,
-                   double* true_res, int* iters) {
+                   double* true_res, int* iters, double src_norm_sq = 0.0,
+                   const double* shift = nullptr) {
     TRY(ensure_work<P>(h, 3));
     const DVec& res = F(h).work[0];
     const DVec& dir = F(h).work[1];
     const DVec& mmp = F(h).work[2];
     const double mass = arg->mass;
-    double src_norm_sq;
-    TRY(norm(h, src, &src_norm_sq));
+    if (src_norm_sq == 0.0) TRY(norm(h, src, &src_norm_sq));
     const double stp_cnd = src_norm_sq * arg->stop_rsd * arg->stop_rsd;
     TRY(mdagm(h, mass, mmp, sol, nullptr));
+    if (shift) TRY(axpby(h, mmp, sol, *shift, 1.0));  // mmp = shift sol + mmp
     TRY(axpy_to(h, res, mmp, src, -1.0, true));  // res = src - mmp
     double rr;
     TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &rr));
@@ -624,6 +629,16 @@ struct Ops {
       itr = itr + 1;
       const double rr_prv = rr;
       double d;
+      if (shift) {
+        bool stop = false;
+        TRY(invcg_shifted_step(h, mass, *shift, sol, dir, res, mmp, stp_cnd, &rr, &d, &stop));
+        if (d == 0.0) {
+          status = DWFGPU_BREAKDOWN;
+          break;
+        }
+        if (stop) break;
+        continue;
+      }
       TRY(mdagm_cg<dwfk::NoX>(h, mass, dir, res, DVec(), &rr, &d));
       if (d == 0.0) {  // a = 0 was used: res is unchanged
         status = DWFGPU_BREAKDOWN;
@@ -643,6 +658,7 @@ struct Ops {
     }
     // true residual |src - A sol| / |src|  (inv_cg.C:403-415)
     TRY(mdagm(h, mass, mmp, sol, nullptr));
+    if (shift) TRY(axpby(h, mmp, sol, *shift, 1.0));
     TRY(axpy_to(h, res, mmp, src, -1.0, true));
     double tr;
     TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, &tr));
@@ -650,6 +666,31 @@ struct Ops {
     if (iters) *iters = itr + 1;
     if (status == DWFGPU_OK && itr == arg->max_num_iter - 1) status = DWFGPU_MAXITER;
     return status;
+  }
+
+  // one step of the shifted CG in CPS's order (inv_cg.C:293-363): mmp = A dir
+  // with d = |MatPc dir|^2, mmp += shift dir, d += shift |dir|^2, then (d != 0)
+  // a = rr / d, sol += a dir, res -= a mmp, rr = |res|^2 and, unless rr meets
+  // the stop rule (*stop), dir = b dir + res.
+  static int invcg_shifted_step(dwfgpu_s* h, double mass, double shift, const DVec& sol,
+                                const DVec& dir, const DVec& res, const DVec& mmp,
+                                double stp_cnd, double* rr, double* d, bool* stop) {
+    const double rr_prv = *rr;
+    TRY(mdagm(h, mass, mmp, dir, d));
+    TRY(axpby(h, mmp, dir, shift, 1.0));
+    double dir_sq;
+    TRY(norm(h, dir, &dir_sq));
+    *d += shift * dir_sq;
+    if (*d == 0.0) return DWFGPU_OK;
+    const double a = rr_prv / *d;
+    TRY(axpby(h, sol, dir, a, 1.0));
+    TRY(axpy_to(h, res, mmp, res, -a, true));
+    TRY(fetch_sum(h, dwfk::BLAS_BLOCKS, rr));
+    if (*rr <= stp_cnd) {
+      *stop = true;
+      return DWFGPU_OK;
+    }
+    return axpby(h, dir, res, 1.0, *rr / rr_prv);
   }
 
   // DiracOpDwf::MatInv (d_op_dwf.C:316-507) / DiracOpMobius::MatInv ORIG
@@ -1580,6 +1621,17 @@ size_t mr_lds(const dwfgpu_s* h, const MrShape& t, size_t creal) {
   return (size_t)2 * 8 * t.ns * (9 * 2 * creal + 4) + (size_t)t.ns * h->ls * t.nr * 8;
 }
 
+// the batched kernels write one partial sum per (RHS, work-group): the
+// buffers hold ceil(vh / MR_MIN_NS) work-groups per RHS, so no shape may use
+// fewer sites per work-group than that
+constexpr int MR_MIN_NS = 8;
+int mr_partials_per_rhs(const dwfgpu_s* h) { return (h->g.vh + MR_MIN_NS - 1) / MR_MIN_NS; }
+
+bool mr_shape_ok(const dwfgpu_s* h, const MrShape& t, int nrhs, size_t creal) {
+  return t.ns >= MR_MIN_NS && t.nr >= 1 && t.nr <= nrhs && t.ns * h->ls * t.nr <= 512 &&
+         mr_lds(h, t, creal) <= (size_t)h->max_lds;
+}
+
 // kind 0: out = D p (cb 1)          kind 1: out = x0 + alpha D in, |out|^2
 // kind 2: out = D^dag in (cb 1)      kind 3: x0 + alpha D^dag in -> CG epilogue
 // kind 4: out = x0 + alpha D^dag in (plain batched MatPcDagMatPc)
@@ -1672,12 +1724,12 @@ std::vector<MrShape> mr_candidates(const dwfgpu_s* h, int nrhs, size_t creal) {
   std::vector<MrShape> c;
   for (int ns : {8, 16, 32})
     for (int nr : {1, 2, 4}) {
-      if (nr > nrhs || ns * h->ls * nr > 512 || ns * h->ls * nr < 64) continue;
       const MrShape t{ns, nr};
-      if (mr_lds(h, t, creal) > (size_t)h->max_lds) continue;
+      if (ns * h->ls * nr < 64 || !mr_shape_ok(h, t, nrhs, creal)) continue;
       c.push_back(t);
     }
-  if (c.empty()) c.push_back(MrShape{h->ns, 1});
+  if (c.empty() && mr_shape_ok(h, MrShape{MR_MIN_NS, 1}, nrhs, creal))
+    c.push_back(MrShape{MR_MIN_NS, 1});
   return c;
 }
 
@@ -1694,15 +1746,23 @@ int mr_tune(dwfgpu_s* h, int nrhs, MrState<P>& m, const int* all_active, MrShape
   const char* env = getenv("DWFGPU_MRHS_SHAPE");  // "ns,nr" pins the shape
   if (env) {
     int ns = 0, nr = 0;
-    if (sscanf(env, "%d,%d", &ns, &nr) == 2 && ns > 0 && nr > 0 && ns * h->ls * nr <= 512) {
-      *best = MrShape{ns, nr};
-      h->mr_ns[pi] = ns;
-      h->mr_nr[pi] = nr;
-      h->mr_nrhs[pi] = nrhs;
-      return DWFGPU_OK;
+    if (sscanf(env, "%d,%d", &ns, &nr) != 2 ||
+        !mr_shape_ok(h, MrShape{ns, nr}, nrhs, sizeof(typename P::R))) {
+      set_err("DWFGPU_MRHS_SHAPE=%s: need ns >= %d, 1 <= nr <= nrhs, ns*Ls*nr <= 512 and the "
+              "LDS of the shape within the device limit", env, MR_MIN_NS);
+      return DWFGPU_BAD_ARG;
     }
+    *best = MrShape{ns, nr};
+    h->mr_ns[pi] = ns;
+    h->mr_nr[pi] = nr;
+    h->mr_nrhs[pi] = nrhs;
+    return DWFGPU_OK;
   }
   const auto cand = mr_candidates(h, nrhs, sizeof(typename P::R));
+  if (cand.empty()) {
+    set_err("mr_tune: no launch shape of the batched operator fits Ls %d", h->ls);
+    return DWFGPU_BAD_ARG;
+  }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -1756,7 +1816,7 @@ int invcg_mrhs(dwfgpu_s* h, const dwfgpu_cg_arg* arg, int nrhs, const double* ma
   const DVec& ax = h->f[DWFGPU_F64].work[1];
   MrState<PS> m;
   m.nrhs = nrhs;
-  m.gx = (g.vh + 8 - 1) / 8;  // partials sized for the smallest candidate ns
+  m.gx = mr_partials_per_rhs(h);  // partials sized for the smallest allowed ns
   TRY(alloc_mvec<PS>(h, nrhs, &m.te));
   TRY(alloc_mvec<PS>(h, nrhs, &m.t1));
   TRY(alloc_mvec<PS>(h, nrhs, &m.rs));
@@ -2891,11 +2951,11 @@ int dwfgpu_dslash(dwfgpu_t h, double mass, double* out, const double* in, int cb
   return DISPATCH(h, to_host(h, out, api_work(h, 1)));
 }
 
-int dwfgpu_invcg(dwfgpu_t h, const dwfgpu_cg_arg* arg, double* sol, const double* src,
-                 double* true_res, int* iters) {
+int dwfgpu_invcg_shift(dwfgpu_t h, const dwfgpu_cg_arg* arg, double* sol, const double* src,
+                       double src_norm_sq, double* true_res, const double* shift, int* iters) {
   if (!h || !arg || !sol || !src) return DWFGPU_BAD_ARG;
-  if (arg->max_num_iter < 1) {
-    set_err("dwfgpu_invcg: max_num_iter must be >= 1");
+  if (arg->max_num_iter < 1 || src_norm_sq < 0.0) {
+    set_err("dwfgpu_invcg: need max_num_iter >= 1 and src_norm_sq >= 0");
     return DWFGPU_BAD_ARG;
   }
   TRY(ensure_api(h, 5));
@@ -2903,10 +2963,15 @@ int dwfgpu_invcg(dwfgpu_t h, const dwfgpu_cg_arg* arg, double* sol, const double
   const DVec& dsrc = api_work(h, 4);
   TRY(DISPATCH(h, to_device(h, dsol, sol)));
   TRY(DISPATCH(h, to_device(h, dsrc, src)));
-  const int st = DISPATCH(h, invcg(h, arg, dsol, dsrc, true_res, iters));
+  const int st = DISPATCH(h, invcg(h, arg, dsol, dsrc, true_res, iters, src_norm_sq, shift));
   if (st != DWFGPU_OK && st != DWFGPU_MAXITER) return st;
   TRY(DISPATCH(h, to_host(h, sol, dsol)));
   return st;
+}
+
+int dwfgpu_invcg(dwfgpu_t h, const dwfgpu_cg_arg* arg, double* sol, const double* src,
+                 double* true_res, int* iters) {
+  return dwfgpu_invcg_shift(h, arg, sol, src, 0.0, true_res, nullptr, iters);
 }
 
 int dwfgpu_matinv(dwfgpu_t h, const dwfgpu_cg_arg* arg, double* out, const double* in,
@@ -2997,11 +3062,20 @@ int dwfgpu_matpcdagmatpc_dev(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_v
   return DWFGPU_OK;
 }
 
+int dwfgpu_invcg_shift_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, dwfgpu_vec_t sol,
+                           dwfgpu_vec_t src, double src_norm_sq, double* true_res,
+                           const double* shift, int* iters) {
+  if (!h || !arg || !sol || !src || sol->h != h || src->h != h) return DWFGPU_BAD_ARG;
+  if (arg->max_num_iter < 1 || src_norm_sq < 0.0) {
+    set_err("dwfgpu_invcg: need max_num_iter >= 1 and src_norm_sq >= 0");
+    return DWFGPU_BAD_ARG;
+  }
+  return DISPATCH(h, invcg(h, arg, sol->v, src->v, true_res, iters, src_norm_sq, shift));
+}
+
 int dwfgpu_invcg_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, dwfgpu_vec_t sol, dwfgpu_vec_t src,
                      double* true_res, int* iters) {
-  if (!h || !arg || !sol || !src || sol->h != h || src->h != h) return DWFGPU_BAD_ARG;
-  if (arg->max_num_iter < 1) return DWFGPU_BAD_ARG;
-  return DISPATCH(h, invcg(h, arg, sol->v, src->v, true_res, iters));
+  return dwfgpu_invcg_shift_dev(h, arg, sol, src, 0.0, true_res, nullptr, iters);
 }
 
 int dwfgpu_invcg_mixed_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, double delta,
@@ -3183,7 +3257,7 @@ int dwfgpu_matpcdagmatpc_mrhs_dev(dwfgpu_t h, int precision, const double* masse
     TRY(ensure_fields<P>(h));
     MrState<P> m;
     m.nrhs = nrhs;
-    m.gx = (h->g.vh + 7) / 8;
+    m.gx = mr_partials_per_rhs(h);
     TRY(alloc_mvec<P>(h, nrhs, &m.te));
     TRY(alloc_mvec<P>(h, nrhs, &m.t1));
     TRY(alloc_mvec<P>(h, nrhs, &m.rs));

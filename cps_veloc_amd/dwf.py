@@ -36,8 +36,9 @@ _lib = None
 EXPORTS = (
     "dwfgpu_create", "dwfgpu_destroy", "dwfgpu_half_size", "dwfgpu_last_error",
     "dwfgpu_matpcdagmatpc", "dwfgpu_matpc", "dwfgpu_dslash", "dwfgpu_invcg",
-    "dwfgpu_matinv", "dwfgpu_vec_create", "dwfgpu_vec_destroy", "dwfgpu_vec_upload",
-    "dwfgpu_vec_download", "dwfgpu_matpcdagmatpc_dev", "dwfgpu_invcg_dev",
+    "dwfgpu_invcg_shift", "dwfgpu_matinv", "dwfgpu_vec_create", "dwfgpu_vec_destroy",
+    "dwfgpu_vec_upload", "dwfgpu_vec_download", "dwfgpu_matpcdagmatpc_dev", "dwfgpu_invcg_dev",
+    "dwfgpu_invcg_shift_dev",
     "dwfgpu_invcg_mixed", "dwfgpu_invcg_mixed_dev",
     "dwfgpu_mvec_create", "dwfgpu_mvec_destroy", "dwfgpu_mvec_upload", "dwfgpu_mvec_download",
     "dwfgpu_matpcdagmatpc_mrhs_dev", "dwfgpu_invcg_mrhs_dev", "dwfgpu_mrhs_shape",
@@ -120,6 +121,12 @@ def lib():
         L.dwfgpu_dslash.argtypes = [H, C.c_double, _dp, _dp, C.c_int, C.c_int]
         L.dwfgpu_invcg.argtypes = [H, C.POINTER(CgArgC), _dp, _dp, C.POINTER(C.c_double),
                                    C.POINTER(C.c_int)]
+        L.dwfgpu_invcg_shift.argtypes = [H, C.POINTER(CgArgC), _dp, _dp, C.c_double,
+                                         C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_int)]
+        L.dwfgpu_invcg_shift_dev.argtypes = [H, C.POINTER(CgArgC), H, H, C.c_double,
+                                             C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                             C.POINTER(C.c_int)]
         L.dwfgpu_matinv.argtypes = [H, C.POINTER(CgArgC), _dp, _dp, C.POINTER(C.c_double),
                                     C.POINTER(C.c_int)]
         L.dwfgpu_vec_create.argtypes = [H, C.POINTER(H)]
@@ -317,6 +324,27 @@ class _DiracOpGpu:
         a = cg_arg.c()
         st = _chk(lib().dwfgpu_invcg(self.h, C.byref(a), out, _arr(inp), C.byref(tr),
                                      C.byref(it)), "InvCg", allow=(OK, MAXITER))
+        return it.value, tr.value, st
+
+    def InvCgShift(self, out, inp, cg_arg: CgArg, src_norm_sq=0.0, shift=None):
+        """DiracOp::InvCgShift(out, in, src_norm_sq, true_res, shift); shift None
+        is CPS's NULL.  Returns (iters, true_res, status)."""
+        tr, it = C.c_double(), C.c_int()
+        a = cg_arg.c()
+        sh = None if shift is None else C.byref(C.c_double(shift))
+        st = _chk(lib().dwfgpu_invcg_shift(self.h, C.byref(a), out, _arr(inp), src_norm_sq,
+                                           C.byref(tr), sh, C.byref(it)), "InvCgShift",
+                  allow=(OK, MAXITER))
+        return it.value, tr.value, st
+
+    def InvCgShift_dev(self, sol: DeviceVector, src: DeviceVector, cg_arg: CgArg,
+                       src_norm_sq=0.0, shift=None):
+        tr, it = C.c_double(), C.c_int()
+        a = cg_arg.c()
+        sh = None if shift is None else C.byref(C.c_double(shift))
+        st = _chk(lib().dwfgpu_invcg_shift_dev(self.h, C.byref(a), sol.h, src.h, src_norm_sq,
+                                               C.byref(tr), sh, C.byref(it)), "InvCgShift_dev",
+                  allow=(OK, MAXITER))
         return it.value, tr.value, st
 
     def MatInv(self, out, inp, cg_arg: CgArg):

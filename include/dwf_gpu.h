@@ -99,6 +99,16 @@ int dwfgpu_dslash(dwfgpu_t h, double mass, double *out, const double *in, int cb
  * iteration cap was hit (solution and true_res still filled).               */
 int dwfgpu_invcg(dwfgpu_t h, const dwfgpu_cg_arg *arg, double *sol, const double *src,
                  double *true_res, int *iters);
+/* DiracOp::InvCgShift(out, in, src_norm_sq, true_res, shift)
+ * (include/util/dirac_op.h:112-116, d_op_base/comsrc/inv_cg.C:84-448), which
+ * also serves InvCg(out, in, src_norm_sq, true_res) (dirac_op.h:118, shift
+ * NULL).  src_norm_sq == 0: |src|^2 is computed (inv_cg.C:166-169); otherwise
+ * the stop rule is |r|^2 <= stop_rsd^2 src_norm_sq and *true_res =
+ * |src - A sol| / sqrt(src_norm_sq).  shift (may be NULL): solve
+ * (MatPcDagMatPc + *shift) sol = src with d = <p, A p> + shift |p|^2
+ * (inv_cg.C:248-251, 304-310).  dwfgpu_invcg is (src_norm_sq 0, shift NULL).*/
+int dwfgpu_invcg_shift(dwfgpu_t h, const dwfgpu_cg_arg *arg, double *sol, const double *src,
+                       double src_norm_sq, double *true_res, const double *shift, int *iters);
 /* DiracOpDwf::MatInv (d_op_dwf.C:316-507) / DiracOpMobius::MatInv
  * (d_op_mobius.C:388-696, ORIG pc) on full vectors; in is preserved
  * (PRESERVE_YES), out holds the initial guess of its CG half on entry.      */
@@ -114,6 +124,9 @@ int dwfgpu_matpcdagmatpc_dev(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_v
                              double *dot_prd);
 int dwfgpu_invcg_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol,
                      dwfgpu_vec_t src, double *true_res, int *iters);
+int dwfgpu_invcg_shift_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol,
+                           dwfgpu_vec_t src, double src_norm_sq, double *true_res,
+                           const double *shift, int *iters);
 
 /* ---- mixed precision ---------------------------------------------------- */
 /* The solve of DiracOp::InvCg (inv_cg.C:84-448: same system, same stop rule

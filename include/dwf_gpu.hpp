@@ -91,6 +91,32 @@ class DiracOpGpu {
     check(dwfgpu_invcg(h_, &a, out, in, true_res, &iters), "InvCg", true);
     return iters;
   }
+  // DiracOp::InvCg(Float *true_res) on the vectors given at construction
+  // (inv_cg.C:599-600; CPS's DWF solve calls it from FdwfBase::FmatEvlInv,
+  // f_dwf_base.C:200): out = f_out, in = f_in of SetVectors
+  int InvCg(double* true_res = nullptr) {
+    if (!f_out_ || !f_in_) check(DWFGPU_BAD_ARG, "InvCg(true_res): no vectors set");
+    return InvCg(f_out_, f_in_, true_res);
+  }
+  // DiracOp::InvCgShift(out, in, src_norm_sq, true_res, shift) (inv_cg.C:84-448)
+  // and InvCg(out, in, src_norm_sq, true_res) (shift nullptr)
+  int InvCgShift(double* out, const double* in, double src_norm_sq, double* true_res,
+                 const double* shift) {
+    int iters = 0;
+    const dwfgpu_cg_arg a = arg_.c();
+    check(dwfgpu_invcg_shift(h_, &a, out, in, src_norm_sq, true_res, shift, &iters),
+          "InvCgShift", true);
+    return iters;
+  }
+  int InvCg(double* out, const double* in, double src_norm_sq, double* true_res) {
+    return InvCgShift(out, in, src_norm_sq, true_res, nullptr);
+  }
+  // the DiracOp constructor's f_field_out / f_field_in (dirac_op_base.C:95-110),
+  // the vectors of InvCg(true_res)
+  void SetVectors(double* f_out, const double* f_in) {
+    f_out_ = f_out;
+    f_in_ = f_in;
+  }
   // DiracOpDwf::MatInv / DiracOpMobius::MatInv (PRESERVE_YES)
   int MatInv(double* out, const double* in, double* true_res = nullptr) {
     int iters = 0;
@@ -166,6 +192,8 @@ class DiracOpGpu {
  private:
   dwfgpu_t h_ = nullptr;
   CgArg arg_;
+  double* f_out_ = nullptr;
+  const double* f_in_ = nullptr;
 };
 
 // DiracOpDwf (src/util/dirac_op/d_op_dwf/d_op_dwf.C)

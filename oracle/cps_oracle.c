@@ -860,6 +860,13 @@ void orc_dslash(double *out, const double *u, const double *in, double mass, int
 int orc_invcg(double *sol, const double *src, const double *u, double mass,
               double stop_rsd, int max_num_iter, double *true_res, double *hist,
               const orc_params *p) {
+  return orc_invcg_shift(sol, src, u, mass, 0.0, NULL, stop_rsd, max_num_iter, true_res, hist,
+                         p);
+}
+
+int orc_invcg_shift(double *sol, const double *src, const double *u, double mass,
+                    double src_norm_sq, const double *shift, double stop_rsd,
+                    int max_num_iter, double *true_res, double *hist, const orc_params *p) {
   const size_t n = orc_half_size(p);
   double *res = (double *)malloc(n * sizeof(double));
   double *dir = (double *)malloc(n * sizeof(double));
@@ -873,9 +880,10 @@ int orc_invcg(double *sol, const double *src, const double *u, double mass,
       orc_mobius_mdagm(o, u, i, d, mass, p, w.t1, w.t2, w.t3);     \
   } while (0)
 
-  double src_norm_sq = orc_norm_sq(src, n);                /* :166-169 */
+  if (src_norm_sq == 0) src_norm_sq = orc_norm_sq(src, n);  /* :166-169 */
   const double stp_cnd = src_norm_sq * stop_rsd * stop_rsd; /* :174 */
   MDAGM(mmp, sol, NULL);                                    /* :248 */
+  if (shift) f_times_v1_plus_v2(mmp, *shift, sol, mmp, n);  /* :249-251 */
   memcpy(res, src, n * sizeof(double));                     /* :255 */
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < (long)n; ++i) res[i] -= mmp[i];      /* :259 */
@@ -890,6 +898,10 @@ int orc_invcg(double *sol, const double *src, const double *u, double mass,
     const double rr_prv = rr;
     double d;
     MDAGM(mmp, dir, &d);
+    if (shift) {                                            /* :304-310 */
+      f_times_v1_plus_v2(mmp, *shift, dir, mmp, n);
+      d += (*shift) * orc_norm_sq(dir, n);
+    }
     if (d == 0.0) {                                         /* :326-331 */
       itr = -itr;
       break;
@@ -908,6 +920,7 @@ int orc_invcg(double *sol, const double *src, const double *u, double mass,
     return 0;
   }
   MDAGM(mmp, sol, NULL);                                    /* :403-415 */
+  if (shift) f_times_v1_plus_v2(mmp, *shift, sol, mmp, n);
   memcpy(res, src, n * sizeof(double));
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < (long)n; ++i) res[i] -= mmp[i];

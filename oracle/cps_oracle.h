@@ -41,6 +41,12 @@
  *   orc_invcg, orc_norm_sq  exactly k+1 iterations on a source spanned by k
  *                           eigenvectors; solution vs dense solve; MAXITER and
  *                           converged-guess edges
+ *   orc_invcg_shift         shifted k-eigenvector source: k+1 iterations and
+ *                           the closed-form solution sum c_i v_i/(lambda_i+s);
+ *                           solution vs the dense solve of (A + s) x = b and
+ *                           vs the pinned multi-shift oracle; src_norm_sq =
+ *                           4|b|^2 at stop_rsd r reproduces |b|^2 at 2r with
+ *                           half the true residual
  *   orc_matinv              full textbook system solved for both actions
  *   orc_fermion5d_canonical_to_full / _full_to_canonical
  *                           against the slice-wise composition of
@@ -139,12 +145,19 @@ void orc_matpc(double *out, const double *u, const double *in, double mass,
 void orc_dslash(double *out, const double *u, const double *in, double mass,
                 int cb, int dag, const orc_params *p);
 
-/* ---- a11: DiracOp::InvCgShift (shift = NULL) --------------------------- */
+/* ---- a11: DiracOp::InvCgShift (d_op_base/comsrc/inv_cg.C:84-448) -------- */
 /* Returns CPS's count itr+1.  hist (may be NULL, length >= max_iter+1)
- * receives |res|^2 after each iteration, hist[0] = initial |res|^2.       */
+ * receives |res|^2 after each iteration, hist[0] = initial |res|^2.
+ * src_norm_sq == 0: |src|^2 is computed (inv_cg.C:166-169); otherwise it sets
+ * the stop rule and the true residual's denominator.  shift (may be NULL):
+ * the solve is (MatPcDagMatPc + *shift) sol = src (inv_cg.C:248-251,304-310).
+ * orc_invcg is InvCgShift(out, in, 0, true_res, NULL).                      */
 int orc_invcg(double *sol, const double *src, const double *u, double mass,
               double stop_rsd, int max_iter, double *true_res, double *hist,
               const orc_params *p);
+int orc_invcg_shift(double *sol, const double *src, const double *u, double mass,
+                    double src_norm_sq, const double *shift, double stop_rsd, int max_iter,
+                    double *true_res, double *hist, const orc_params *p);
 
 /* ---- a12: BLAS-1 ------------------------------------------------------- */
 double orc_norm_sq(const double *a, size_t n);   /* Vector::NormSqNode, serial */

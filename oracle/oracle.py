@@ -53,6 +53,10 @@ def lib():
         L.orc_matinv.restype = C.c_int
         L.orc_matinv.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double, C.c_int,
                                  C.POINTER(C.c_double), P]
+        L.orc_invcg_shift.restype = C.c_int
+        L.orc_invcg_shift.argtypes = [_dp, _dp, _dp, C.c_double, C.c_double,
+                                      C.POINTER(C.c_double), C.c_double, C.c_int,
+                                      C.POINTER(C.c_double), _dp, P]
         _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
         L.orc_minvcg.restype = C.c_int
         L.orc_minvcg.argtypes = [_dp, _dp, C.c_double, _dp, C.c_int, _dp, C.c_int, _dp, C.c_int,
@@ -139,6 +143,18 @@ class Oracle:
         hist = np.zeros(max_iter + 2)
         it = lib().orc_invcg(sol, np.ascontiguousarray(src, np.float64), self.u, mass,
                              stop_rsd, max_iter, C.byref(tr), hist, C.byref(self.p))
+        return sol, it, tr.value, hist[: max(it, 1)]
+
+    def invcg_shift(self, src, mass, src_norm_sq=0.0, shift=None, stop_rsd=1e-10,
+                    max_iter=10000, guess=None):
+        """InvCgShift(out, in, src_norm_sq, true_res, shift); shift None = NULL."""
+        sol = np.zeros(self.n) if guess is None else np.array(guess, np.float64)
+        tr = C.c_double(0.0)
+        hist = np.zeros(max_iter + 2)
+        sh = None if shift is None else C.byref(C.c_double(shift))
+        it = lib().orc_invcg_shift(sol, np.ascontiguousarray(src, np.float64), self.u, mass,
+                                   src_norm_sq, sh, stop_rsd, max_iter, C.byref(tr), hist,
+                                   C.byref(self.p))
         return sol, it, tr.value, hist[: max(it, 1)]
 
     def matinv(self, src_full, mass, stop_rsd=1e-10, max_iter=10000, guess=None):

@@ -51,29 +51,40 @@ def transport(phi, U, d):
     mu = (d - 1) % 4
     ax = AXIS[mu]
     Umu = U[..., mu, :, :]  # [t, z, y, x, 3, 3]
+    # out[c, x, s, a] = sum_b M[x, a, b] nb[c, x, s, b] = (nb @ M^T)[c, x, s, a]
     if d <= 4:
         nb = np.roll(phi, -1, axis=ax)  # phi(x + mu)
-        return np.einsum("tzyxab,ctzyxsb->ctzyxsa", Umu, nb)
+        return np.matmul(nb, np.swapaxes(Umu, -1, -2))  # M = U_mu(x)
     nb = np.roll(phi, 1, axis=ax)  # phi(x - mu)
     Ub = np.roll(Umu, 1, axis=ax - 1)  # U_mu(x - mu)
-    return np.einsum("tzyxba,ctzyxsb->ctzyxsa", Ub.conj(), nb)
+    return np.matmul(nb, Ub.conj())  # M = U_mu(x - mu)^dag, M^T = conj(U)
+
+
+def _spin_mix(W, f):
+    """out[o, x, a, c] = sum_{i, b} W[o, i, a, b] f[i, x, b, c] (a BLAS
+    contraction over (i, b))."""
+    return np.moveaxis(np.tensordot(W, f, axes=([1, 3], [0, 5])), 1, 5)
 
 
 def layer(W, phi, U):
-    """W[d, o, i, 4, 4] complex."""
-    out = 0
-    for d in range(NDIR):
-        out = out + np.einsum("oiab,itzyxbc->otzyxac", W[d], transport(phi, U, d))
+    """W[d, o, i, 4, 4] complex: sum_d W_d (T_d phi)."""
+    out = _spin_mix(W[0], phi)
+    for d in range(1, NDIR):
+        out += _spin_mix(W[d], transport(phi, U, d))
     return out
 
 
 def layer_adjoint(W, g, U):
     """The adjoint map: sum_d T_{-d} (W_d^dag g), so <g, layer(phi)> = <layer_adjoint(g), phi>."""
-    out = 0
+    out = None
     for d in range(NDIR):
         dm = 0 if d == 0 else (d + 4 if d <= 4 else d - 4)
         Wd = np.conj(np.transpose(W[d], (1, 0, 3, 2)))  # [i, o, b, a]
-        out = out + transport(np.einsum("ioab,otzyxbc->itzyxac", Wd, g), U, dm)
+        t = transport(_spin_mix(Wd, g), U, dm)
+        if out is None:
+            out = t
+        else:
+            out += t
     return out
 
 
@@ -84,7 +95,9 @@ def weight_grad(g, phi, U):
     G = []
     for d in range(NDIR):
         t = transport(phi, U, d)
-        G.append(2.0 * np.einsum("otzyxac,itzyxbc->oiab", g, np.conj(t)))
+        # sum over the sites and colour: [o, a, i, b] -> [o, i, a, b]
+        G.append(2.0 * np.tensordot(g, np.conj(t), axes=([1, 2, 3, 4, 6], [1, 2, 3, 4, 6]))
+                 .transpose(0, 2, 1, 3))
     return np.array(G)
 
 

@@ -2,6 +2,8 @@
 // DiracOpDwf / DiracOpMobius, and checks it against the C ABI:
 //   MatPcDag(MatPc(x)) == MatPcDagMatPc(x) bit for bit,
 //   InvCg through the facade == dwfgpu_invcg (same iterations, same bits),
+//   InvCg(true_res) on the constructor's vectors == InvCg(out, in, true_res),
+//   a caller-supplied src_norm_sq and a shifted solve (InvCgShift),
 //   |src - A sol| / |src| recomputed here == the returned true residual,
 //   MAXITER is reported as a count, a bad argument as dwfgpu::Error;
 //   MInvCG solves every shift, ImpResLanczos returns eigenpairs, the
@@ -67,6 +69,36 @@ static int run(const char* name) {
   for (size_t k = 0; k < n; ++k) a[k] = x[k] - a[k];
   const double tr = std::sqrt(norm2(a) / norm2(x));
   REQUIRE(tr <= 1e-10 && std::fabs(tr - tr1) <= 1e-6 * tr1);
+
+  // InvCg(Float *true_res) on the constructor's vectors, the form CPS's DWF
+  // solve calls (FdwfBase::FmatEvlInv, f_dwf_base.C:200): the same solve
+  std::vector<double> s4(n, 0.0);
+  op.SetVectors(s4.data(), x.data());
+  double tr4 = 0;
+  REQUIRE(op.InvCg(&tr4) == it1 && tr4 == tr1);
+  REQUIRE(std::memcmp(s4.data(), s1.data(), n * sizeof(double)) == 0);
+  // InvCg(out, in, src_norm_sq, true_res): src_norm_sq = 4 |x|^2 at stop_rsd
+  // r is the computed-norm solve at 2r, with half its true residual
+  dwfgpu::CgArg wide = arg;
+  wide.stop_rsd = 2e-8;
+  op.DiracArg(wide);
+  std::vector<double> s5(n, 0.0), s6(n, 0.0);
+  double tr5 = 0, tr6 = 0;
+  const int it6 = op.InvCg(s6.data(), x.data(), &tr6);
+  wide.stop_rsd = 1e-8;
+  op.DiracArg(wide);
+  REQUIRE(op.InvCg(s5.data(), x.data(), 4.0 * norm2(x), &tr5) == it6);
+  REQUIRE(std::fabs(tr5 - 0.5 * tr6) <= 1e-12 * tr6);
+  op.DiracArg(arg);
+  // InvCgShift: (A + sigma) sol = x
+  const double sigma = 0.05;
+  std::vector<double> s7(n, 0.0);
+  double tr7 = 0;
+  const int it7 = op.InvCgShift(s7.data(), x.data(), 0.0, &tr7, &sigma);
+  op.MatPcDagMatPc(a.data(), s7.data());
+  for (size_t q = 0; q < n; ++q) a[q] = x[q] - a[q] - sigma * s7[q];
+  REQUIRE(it7 > 1 && it7 < it1 && tr7 <= 1e-10);
+  REQUIRE(std::fabs(std::sqrt(norm2(a) / norm2(x)) - tr7) <= 1e-6 * tr7);
 
   dwfgpu::CgArg few = arg;
   few.max_num_iter = 3;

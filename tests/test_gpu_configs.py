@@ -1,0 +1,251 @@
+"""GPU parity at the production workloads: the HIP path through its C ABI
+against the CPU oracle at the full lattice sizes the solvers are run at.
+
+  * Mobius 24^3x64, Ls 12, b 1.5, c 0.5, M5 1.8, mf 0.005, links eps 0.4:
+    FP64 MatPcDagMatPc, MatPc and MatPcDag within 1e-13 per site of the
+    oracle (27,648 work-groups through the XCD remap, the 8 t x 4 z tile, the
+    non-temporal and backwards-swept passes); the FP32 and int16 handles
+    within the reduced-precision bounds of tests/test_gpu_mixed.py.
+  * DWF 16^3x32, Ls 16, mf 0.01: FP64 operator at 1e-13 per site; the oracle's
+    FP64 InvCg solve to 1e-10 against the GPU plain CG (same iteration count)
+    and the mixed FP32 / BF16 / int16 solves (FP64 true residual of the oracle
+    operator <= 1e-10; solution error against the oracle solve printed and
+    bounded by 1e-8 relative; iterations: FP32 and int16 within 3 of the
+    oracle's count, BF16 within the stated loss bound).
+  * the 24-RHS batched workload at 16^3x32, Ls 16 (12 point + 12 Z2 sources
+    prepared as DiracOpDwf::MatInv does, masses 0.005 / 0.01 / 0.04): the
+    batched FP64 operator against the oracle for all 24 RHS, the batched FP32
+    solve with every RHS's residual checked by the oracle operator, the
+    single-RHS count within 2 per RHS, and one RHS against an oracle solve.
+  * the network preconditioner at 16^3x32, Ls 16 (Mobius): forward and weight
+    gradients against oracle/ptc_net.py within the BF16 bounds of
+    tests/test_gpu_ptc.py.
+The oracle runs on the host cores (OpenMP); these tests take minutes.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cps_veloc_amd import lattice as lat
+from cps_veloc_amd.dwf import CgArg, DiracOpDwfGpu, DiracOpMobiusGpu
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+M5 = 1.8
+C4 = dict(dims=(24, 24, 24, 64), ls=12, mass=0.005, eps=0.4, b=1.5, c=0.5)
+C2 = dict(dims=(16, 16, 16, 32), ls=16, mass=0.01, eps=0.3)
+OP_TOL = {"f32": 1e-5, "bf16": 2e-2, "i16": 2e-4}
+# BF16 spinors (8-bit significand) cost iterations in the sloppy recurrence;
+# the bound states today's loss on this workload
+BF16_ITER_FACTOR = 1.5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def site_rel(got, want):
+    g = np.asarray(got).reshape(-1, 24)
+    w = np.asarray(want).reshape(-1, 24)
+    nw = np.linalg.norm(w, axis=1)
+    return float(np.max(np.linalg.norm(g - w, axis=1) / np.maximum(nw, 1e-300)))
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def src_vec(dims, ls, seed):
+    return lat.to_reals(lat.gaussian_spinor((ls, int(np.prod(dims)) // 2), seed))
+
+
+@pytest.mark.timeout(600)
+def test_mobius_24x24x24x64_ls12_operator():
+    c = C4
+    say(f"[gpu] Mobius {c['dims']} ls={c['ls']} mf={c['mass']}: FP64 operator vs oracle")
+    U = lat.random_gauge(c["dims"], c["eps"], 1234)
+    x = src_vec(c["dims"], c["ls"], 5678)
+    o = Oracle(c["dims"], c["ls"], "mobius", m5=M5, b=c["b"], c=c["c"], gauge_canonical=U)
+    want, dwant = o.matpcdagmatpc(x, c["mass"])
+    g = DiracOpMobiusGpu(U, c["dims"], c["ls"], m5=M5, b=c["b"], c=c["c"])
+    got = np.zeros(g.n)
+    d = g.MatPcDagMatPc(got, x, c["mass"])
+    err = site_rel(got, want)
+    say(f"  MatPcDagMatPc: max site rel err {err:.2e}, dot rel err {abs(d - dwant) / dwant:.2e}")
+    assert err < 1e-13 and abs(d - dwant) / dwant < 1e-13
+    for dag in (0, 1):
+        wpc = o.matpc(x, c["mass"], dag)
+        (g.MatPcDag if dag else g.MatPc)(got, x, c["mass"])
+        err = site_rel(got, wpc)
+        say(f"  MatPc dag={dag}: max site rel err {err:.2e}")
+        assert err < 1e-13
+    g.close()
+    for prec in ("f32", "i16"):
+        gp = DiracOpMobiusGpu(U, c["dims"], c["ls"], m5=M5, b=c["b"], c=c["c"], precision=prec)
+        gp.MatPcDagMatPc(got, x, c["mass"])
+        err = rel(got, want)
+        say(f"  {prec} handle MatPcDagMatPc: normwise rel err {err:.2e}")
+        assert err < OP_TOL[prec]
+        gp.close()
+
+
+@pytest.mark.timeout(900)
+def test_dwf_16x16x16x32_ls16_solves_vs_oracle_solve():
+    c = C2
+    dims, ls, mass = c["dims"], c["ls"], c["mass"]
+    say(f"[gpu] DWF {dims} ls={ls} mf={mass}: operator, plain and mixed CG vs oracle solve")
+    U = lat.random_gauge(dims, c["eps"], 1234)
+    b = src_vec(dims, ls, 5678)
+    o = Oracle(dims, ls, "dwf", m5=M5, gauge_canonical=U)
+    g = DiracOpDwfGpu(U, dims, ls, m5=M5)
+    want, dwant = o.matpcdagmatpc(b, mass)
+    got = np.zeros(g.n)
+    d = g.MatPcDagMatPc(got, b, mass)
+    assert site_rel(got, want) < 1e-13 and abs(d - dwant) / dwant < 1e-13
+    for dag in (0, 1):
+        (g.MatPcDag if dag else g.MatPc)(got, b, mass)
+        assert site_rel(got, o.matpc(b, mass, dag)) < 1e-13
+    say("  oracle FP64 CG solve ...")
+    xo, oit, otr, _ = o.invcg(b, mass, stop_rsd=1e-10, max_iter=5000)
+    say(f"  oracle: {oit} iterations, true res {otr:.3e}")
+    arg = CgArg(mass, 5000, 1e-10)
+    xp = np.zeros(g.n)
+    pit, ptr, pst = g.InvCg(xp, b, arg)
+    say(f"  GPU plain CG: {pit} iterations, true res {ptr:.3e}, err vs oracle {rel(xp, xo):.2e}")
+    assert pst == 0 and pit == oit and rel(xp, xo) < 1e-8
+    nb = np.linalg.norm(b)
+    for sloppy in ("f32", "bf16", "i16"):
+        x = np.zeros(g.n)
+        it, tr, st, nrel = g.InvCgMixed(x, b, arg, sloppy, 0.1)
+        ax, _ = o.matpcdagmatpc(x, mass)
+        otrue = float(np.linalg.norm(b - ax) / nb)
+        err = rel(x, xo)
+        say(f"  mixed {sloppy}: {it} iterations ({nrel} reliable updates), true res "
+            f"{tr:.3e} (oracle operator {otrue:.3e}), |x - x_oracle|/|x_oracle| {err:.2e}")
+        assert st == 0 and tr <= 1e-10 and otrue <= 1.0001e-10
+        assert err < 1e-8
+        if sloppy == "bf16":
+            assert it <= BF16_ITER_FACTOR * oit
+        else:
+            assert it <= oit + 3
+
+
+def _mrhs_sources(o, dims, ls, masses, kappa, seed=99):
+    """The 24 CG sources: full 5D point (spin-colour at the origin, P+ on
+    s = 0, P- on s = Ls-1, as Ffour2five(src, 0, Ls-1)) and complex Z2
+    sources, prepared as DiracOpDwf::MatInv does (d_op_dwf.C:357-419):
+    b = MatPcDag(eta_odd + kappa Dslash(eta_even)), by the oracle."""
+    rng = np.random.default_rng(seed)
+    vh = int(np.prod(dims)) // 2
+    srcs = []
+    for j in range(24):
+        mass = masses[j]
+        ev = np.zeros((ls, vh, 4, 3, 2))
+        od = np.zeros((ls, vh, 4, 3, 2))
+        if j < 12:
+            spin, col = divmod(j, 3)
+            if spin < 2:
+                ev[0, 0, spin, col, 0] = 1.0
+            else:
+                od[ls - 1, 0, spin, col, 0] = 1.0
+        else:
+            ev = rng.choice([-1.0, 1.0], size=ev.shape) / np.sqrt(2.0)
+            od = rng.choice([-1.0, 1.0], size=od.shape) / np.sqrt(2.0)
+        t = od.reshape(-1) + kappa * o.dslash(ev.reshape(-1), mass, 0, 0)
+        srcs.append(o.matpc(t, mass, 1))
+    return srcs
+
+
+@pytest.mark.timeout(900)
+def test_dwf_16x16x16x32_ls16_mrhs24_vs_oracle():
+    c = C2
+    dims, ls = c["dims"], c["ls"]
+    masses = np.array([(0.005, 0.01, 0.04)[j % 3] for j in range(24)])
+    say(f"[gpu] 24-RHS batched DWF {dims} ls={ls}: operator and solve vs oracle")
+    U = lat.random_gauge(dims, c["eps"], 1234)
+    o = Oracle(dims, ls, "dwf", m5=M5, gauge_canonical=U)
+    g = DiracOpDwfGpu(U, dims, ls, m5=M5)
+    kappa = 1.0 / (2.0 * (4.0 + 1.0 - M5))
+    srcs = _mrhs_sources(o, dims, ls, masses, kappa)
+    vin, vout = g.multi_vector(24, srcs), g.multi_vector(24)
+    g.MatPcDagMatPc_mrhs_dev(vout, vin, masses, "f64")
+    worst = 0.0
+    for j in range(24):
+        want, _ = o.matpcdagmatpc(srcs[j], masses[j])
+        worst = max(worst, site_rel(vout.download(j), want))
+    say(f"  batched FP64 operator, 24 RHS: max site rel err {worst:.2e}")
+    assert worst < 1e-13
+    arg = CgArg(0.0, 5000, 1e-10)
+    sol = g.multi_vector(24)
+    it, tr, st, nrel = g.InvCgMrhs_dev(sol, vin, arg, masses, "f32", 0.1)
+    say(f"  batched f32 CG iterations {it.tolist()}")
+    assert np.all(st == 0) and np.all(tr <= 1e-10)
+    v1, s1 = g.vector(), g.vector()
+    for j in range(24):
+        x = sol.download(j)
+        ax, _ = o.matpcdagmatpc(x, masses[j])
+        otrue = float(np.linalg.norm(srcs[j] - ax) / np.linalg.norm(srcs[j]))
+        assert otrue <= 1.0001e-10, (j, otrue)
+        s1.upload(srcs[j])
+        v1.upload(np.zeros(g.n))
+        i1, _, st1, _ = g.InvCgMixed_dev(v1, s1, CgArg(masses[j], 5000, 1e-10), "f32", 0.1)
+        assert st1 == 0 and abs(int(it[j]) - i1) <= 2, (j, it[j], i1)
+    j = 2  # a point source at mf 0.04: the cheapest oracle solve
+    xo, oit, _, _ = o.invcg(srcs[j], masses[j], stop_rsd=1e-10, max_iter=5000)
+    err = rel(sol.download(j), xo)
+    say(f"  RHS {j}: batched {it[j]} iterations, oracle {oit}; |x - x_oracle|/|x_oracle| {err:.2e}")
+    assert err < 1e-8 and abs(int(it[j]) - oit) <= 3
+
+
+HID = 16
+
+
+def _bf16_round(x):
+    a = np.ascontiguousarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.timeout(900)
+def test_network_16x16x16x32_ls16_vs_oracle():
+    from oracle import ptc_net as pn
+    dims, ls = (16, 16, 16, 32), 16
+    say(f"[gpu] network forward and gradients, Mobius {dims} ls={ls}")
+    U = lat.random_gauge(dims, 0.3, 1234)
+    g = DiracOpMobiusGpu(U, dims, ls, m5=M5, b=1.5, c=0.5)
+    g.ptc_init(1, 0.02)
+    rng = np.random.default_rng(7)
+    n = g.ptc_num_weights()
+    w = (0.2 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    w = (_bf16_round(w.real) + 1j * _bf16_round(w.imag)).astype(np.complex64)
+    g.ptc_set_weights(w)
+    shapes = [(9, HID, ls, 4, 4), (9, HID, HID, 4, 4), (9, ls, HID, 4, 4)]
+    Ws, off = [], 0
+    for sh in shapes:
+        k = int(np.prod(sh))
+        Ws.append(np.asarray(w[off:off + k]).reshape(sh).astype(np.complex128))
+        off += k
+    links = pn.links_with_bc(U, dims)
+    r = src_vec(dims, ls, 9)
+    e = src_vec(dims, ls, 10)
+    got = g.ptc_apply(r)
+    want = pn.apply_net(Ws, r, links, dims, ls, "mobius")
+    err = rel(got, want)
+    say(f"  forward: normwise rel err {err:.3e}")
+    assert err < 3e-2
+    loss, grad = g.ptc_loss_grad(r, e)
+    wl, wg = pn.loss_and_grads_cg(Ws, r, e, links, dims, ls, "mobius")
+    say(f"  loss gpu {loss:.6e} oracle {wl:.6e}")
+    assert abs(loss - wl) < 3e-2 * wl
+    off = 0
+    for k, sh in enumerate(shapes):
+        cnt = int(np.prod(sh))
+        gk = np.asarray(grad[off:off + cnt]).reshape(sh)
+        off += cnt
+        gerr = rel(gk, wg[k])
+        say(f"  layer {k}: gradient normwise rel err {gerr:.3e}")
+        assert gerr < 6e-2

@@ -11,7 +11,9 @@ Bars
   * the mixed CG must reach the FP64 true residual the plain CG reaches
     (|src - A sol| <= stop_rsd |src|, checked from an independent FP64
     MatPcDagMatPc), and its solution must agree with the oracle's solution to
-    the accuracy the stop rule allows (1e-8 relative, as for the plain CG).
+    the accuracy the stop rule allows (1e-8 relative, as for the plain CG);
+    FP32 and int16 within 3 iterations of the oracle's count, BF16 within
+    1.5 times it.
 """
 import sys
 
@@ -26,6 +28,9 @@ pytestmark = pytest.mark.gpu
 
 M5 = 1.8
 OP_TOL = {"f32": 1e-5, "bf16": 2e-2, "i16": 2e-4}
+# BF16 sloppy spinors: the iteration count the mixed CG may take, relative to
+# the FP64 count (the loss this workload shows)
+BF16_ITER_FACTOR = 1.5
 
 
 def say(*a):
@@ -114,7 +119,12 @@ def test_mixed_cg(sloppy, action, dims, ls, mass):
     say(f"  oracle iters={oit}  mixed iters={it} reliable updates={nrel} true_res={tr:.3e}")
     assert st == 0
     assert tr <= 1e-10 and nrel >= 1
-    assert it <= 3 * oit
+    # FP32 and int16 keep the oracle's count (plus the two steps by which the
+    # host's lagged test takes the stop); BF16 loses conjugacy between updates
+    if sloppy == "bf16":
+        assert it <= BF16_ITER_FACTOR * oit
+    else:
+        assert it <= oit + 3
     # independent check of the true residual with the FP64 operator
     ax = np.zeros(g.n)
     g.MatPcDagMatPc(ax, sol, mass)

@@ -222,6 +222,56 @@ def test_cg_iterations_equal_eigen_count(action):
 
 
 @pytest.mark.parametrize("action", ["dwf", "mobius"])
+def test_cg_shift_eigen_count_and_closed_form(action):
+    """InvCgShift with a shift s (inv_cg.C:248-251,304-310): (A + s) has A's
+    eigenvectors, so a source spanned by k of them ends after k iterations
+    with the solution sum c_i v_i / (lambda_i + s)."""
+    dims, ls, mass, s = (2, 2, 2, 2), 4, 0.1, 0.37
+    U, o = setup(dims, ls, action)
+    A = _dense_mdagm(o, mass)
+    w, V = np.linalg.eigh(A)
+    for k in (1, 2, 3):
+        idx = [7, 90, 210][:k]
+        coef = np.array([1.0, -0.5, 2.0])[:k]
+        src = V[:, idx] @ coef
+        sol, it, tr, _ = o.invcg_shift(src, mass, shift=s, stop_rsd=1e-9, max_iter=50)
+        assert it == k + 1, (k, it)
+        want = V[:, idx] @ (coef / (w[idx] + s))
+        assert rel(sol, want) < 1e-12
+        assert tr < 1e-9
+    # shift NULL and shift 0 are the unshifted CG
+    src = V[:, [3, 50, 120, 300]].sum(axis=1)
+    a = o.invcg(src, mass, stop_rsd=1e-10)
+    b = o.invcg_shift(src, mass, shift=0.0, stop_rsd=1e-10)
+    assert a[1] == b[1] and rel(b[0], a[0]) < 1e-14
+
+
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
+def test_cg_shift_solution_and_src_norm_sq(action):
+    """The shifted solve against the dense solve of (A + s) x = b and the
+    pinned multi-shift oracle; a caller-supplied src_norm_sq sets the stop
+    rule stop_rsd^2 src_norm_sq and the true-residual denominator
+    (inv_cg.C:166-174, 403-415)."""
+    dims, ls, mass, s = (2, 2, 2, 4), 4, 0.05, 0.02
+    U, o = setup(dims, ls, action)
+    vh = int(np.prod(dims)) // 2
+    b = lat.to_reals(lat.gaussian_spinor((ls, vh), 43))
+    A = _dense_mdagm(o, mass)
+    x, it, tr, _ = o.invcg_shift(b, mass, shift=s, stop_rsd=1e-12, max_iter=3000)
+    want = np.linalg.solve(A + s * np.eye(o.n), b)
+    assert rel(x, want) < 1e-10
+    r = b - A @ x - s * x
+    assert abs(tr - np.linalg.norm(r) / np.linalg.norm(b)) < 1e-14
+    psi, _, _, its = o.minvcg(b, mass, [s], 1e-12)
+    assert rel(x, psi[0]) < 1e-10 and abs(its[0] - (it - 1)) <= 1
+    nb2 = float(np.dot(b, b))
+    x1, it1, tr1, _ = o.invcg_shift(b, mass, src_norm_sq=4 * nb2, stop_rsd=1e-9)
+    x2, it2, tr2, _ = o.invcg_shift(b, mass, src_norm_sq=0.0, stop_rsd=2e-9)
+    assert it1 == it2 and np.array_equal(x1, x2)
+    assert abs(tr1 - 0.5 * tr2) < 1e-15
+
+
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
 def test_cg_solution_and_edges(action):
     dims, ls, mass = (4, 4, 2, 2), 4, 0.05
     U, o = setup(dims, ls, action)
