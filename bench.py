@@ -51,7 +51,7 @@ def log(*a):
 
 
 def traffic_from_profile(workload: str, precision: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprof
+    """Memory traffic per launch of the hop kernels from the committed rocprof
     PMC summary (profiles/<workload>_<precision>_traffic.json), if present."""
     path = os.path.join(ROOT, "profiles", f"{workload}_{precision}_traffic.json")
     if not os.path.exists(path):
@@ -60,17 +60,84 @@ def traffic_from_profile(workload: str, precision: str):
         return json.load(f)
 
 
-def cpu_baseline(wl, U, src, seconds_hint=True):
-    """The oracle (C restatement of CPS's noarch path) timed on the host."""
-    from oracle.oracle import Oracle
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def site_errors(got, want, dgot=None, dwant=None):
+    g, w = np.asarray(got).reshape(-1, 24), np.asarray(want).reshape(-1, 24)
+    nw = np.maximum(np.linalg.norm(w, axis=1), 1e-300)
+    out = {"max_site_rel_err": float(np.max(np.linalg.norm(g - w, axis=1) / nw))}
+    if dgot is not None:
+        out["dot_rel_err"] = float(abs(dgot - dwant) / abs(dwant))
+    return out
+
+
+def cpu_baseline(wl, U, src):
+    """CPS's CPU path as the oracle restates it (oracle/cps_oracle.c), timed on
+    the host cores: one MatPcDagMatPc of the bench workload with the 4D hop in
+    one thread as in CPS's noarch wilson_dslash.C:280-477 (the 5D terms and
+    vector arithmetic on OpenMP threads, as CPS's *_omp.C / vector_util.C), and
+    one with the hop spread over OpenMP threads.  Returns the timings and the
+    output of the serial run (the same bits as the parallel one)."""
+    from oracle import oracle as orc
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = Oracle(wl["dims"], wl["ls"], wl["action"], m5=1.8, b=wl.get("b", 1.5),
-               c=wl.get("c", 0.5), gauge_canonical=U)
+    o = orc.Oracle(wl["dims"], wl["ls"], wl["action"], m5=1.8, b=wl.get("b", 1.5),
+                   c=wl.get("c", 0.5), gauge_canonical=U)
     o.matpcdagmatpc(src, wl["mass"])  # warm (page faults, OpenMP start)
     t0 = time.perf_counter()
     o.matpcdagmatpc(src, wl["mass"])
-    dt = time.perf_counter() - t0
-    return dt, cores
+    t_par = time.perf_counter() - t0
+    orc.set_serial_hop(True)
+    try:
+        t0 = time.perf_counter()
+        want, dwant = o.matpcdagmatpc(src, wl["mass"])
+        t_ser = time.perf_counter() - t0
+    finally:
+        orc.set_serial_hop(False)
+    return {"serial_hop_s": t_ser, "omp_s": t_par, "cores": cores, "out": want, "dot": dwant}
+
+
+def c1_solves():
+    """The 8^3x8, Ls 8 DWF InvCg (mf 0.04, stop 1e-10): the CPS-faithful CPU
+    solve (serial 4D hop) and the GPU's plain FP64 CG on the same system."""
+    from cps_veloc_amd import lattice as lat
+    from cps_veloc_amd.dwf import CgArg, DiracOpDwfGpu
+    from oracle import oracle as orc
+    wl = WORKLOADS["dwf_8x8x8x8_ls8"]
+    dims, ls, mass = wl["dims"], wl["ls"], wl["mass"]
+    U = lat.random_gauge(dims, wl["eps"], 1234)
+    src = lat.to_reals(lat.gaussian_spinor((ls, int(np.prod(dims)) // 2), 5678))
+    o = orc.Oracle(dims, ls, "dwf", m5=1.8, gauge_canonical=U)
+    orc.set_serial_hop(True)
+    try:
+        t0 = time.perf_counter()
+        xo, oit, otr, _ = o.invcg(src, mass, stop_rsd=1e-10, max_iter=5000)
+        t_cpu = time.perf_counter() - t0
+    finally:
+        orc.set_serial_hop(False)
+    op = DiracOpDwfGpu(U, dims, ls, m5=1.8)
+    vin, vsol = op.vector(src), op.vector()
+    op.InvCg_dev(vsol, vin, CgArg(mass, 3, 1e-10))  # first-use allocations
+    vsol.upload(np.zeros(op.n))
+    t0 = time.perf_counter()
+    git, gtr, gst = op.InvCg_dev(vsol, vin, CgArg(mass, 5000, 1e-10))
+    t_gpu = time.perf_counter() - t0
+    err = float(np.linalg.norm(vsol.download() - xo) / np.linalg.norm(xo))
+    vin.free()
+    vsol.free()
+    op.close()
+    return {"workload": "dwf_8x8x8x8_ls8", "mass": mass, "stop_rsd": 1e-10,
+            "cpu_seconds": round(t_cpu, 3), "cpu_iters": oit, "cpu_true_res": otr,
+            "gpu_seconds": round(t_gpu, 4), "gpu_iters": git, "gpu_true_res": gtr,
+            "gpu_status": gst, "rel_err_gpu_vs_cpu_solution": err}
 
 
 def mrhs_sources(op, wl, kappa, seed=99):
@@ -216,11 +283,17 @@ def main():
     gflops = flops_app / (ms_app * 1e-3) / 1e9
     log(f"[bench] MatPcDagMatPc {ms_app:.3f} ms  {gbs:.0f} GB/s  {gflops:.0f} GFLOP/s "
         f"(wall {wall:.2f}s); kernels ms {np.round(ms_k, 4).tolist()}")
+    v_gpu = vout.download()  # MatPcDagMatPc(src), checked against the oracle below
+    d_gpu = op.MatPcDagMatPc(np.zeros(op.n), src, mass)
     hop = int(np.argmax(ms_k[:4]))
     k_gbs = bytes_k[hop] / (ms_k[hop] * 1e-3) / 1e9
     copy_gbs = op.time_copy(2 * 1024 ** 3, 10)
-    log(f"[bench] dominant kernel #{hop}: {k_gbs:.0f} GB/s; copy kernel {copy_gbs:.0f} GB/s")
+    bw_copy, bw_read, bw_r3w1 = op.time_bandwidth(2 * 1024 ** 3, 10)
+    log(f"[bench] dominant kernel #{hop}: {k_gbs:.0f} GB/s; copy kernel {copy_gbs:.0f} GB/s; "
+        f"probes: copy {bw_copy:.0f}, read-only {bw_read:.0f}, 3 reads + 1 write "
+        f"{bw_r3w1:.0f} GB/s")
     prof = traffic_from_profile(args.workload, "f64")
+    kprof = {} if prof is None else prof.get("kernels", {}).get(KERNELS[wl["action"]][hop], {})
 
     sloppy = {}
     for p in SLOPPY:
@@ -302,15 +375,35 @@ def main():
                              "speedup_vs_plain_fp64": round(t_plain / best[1], 3)}
         vsol.free()
 
-    cpu = None
+    cpu, parity = None, None
     if not args.no_cpu:
-        log("[bench] CPU oracle: one MatPcDagMatPc application")
-        dt, cores = cpu_baseline(wl, U, src)
-        cpu = {"value": round(bytes_app / dt / 1e9, 3), "unit": "GB/s", "cores": cores,
+        log("[bench] CPU baseline: one MatPcDagMatPc application, serial 4D hop and OpenMP hop")
+        cb = cpu_baseline(wl, U, src)
+        parity = site_errors(v_gpu, cb["out"], d_gpu, cb["dot"])
+        parity["against"] = "oracle/cps_oracle.c MatPcDagMatPc of the same source (FP64)"
+        log(f"[bench] GPU vs oracle: max site rel err {parity['max_site_rel_err']:.2e}, "
+            f"dot rel err {parity['dot_rel_err']:.2e}")
+        log("[bench] CPU baseline: 8^3x8 Ls 8 full solve (serial 4D hop) and the GPU's")
+        c1 = c1_solves()
+        t_ser, t_par, cores = cb["serial_hop_s"], cb["omp_s"], cb["cores"]
+        cpu = {"value": round(bytes_app / t_ser / 1e9, 3), "unit": "GB/s", "cores": cores,
                "kind": "port",
-               "sample": f"1 MatPcDagMatPc application of the same workload in "
-                         f"{dt:.2f}s (oracle/cps_oracle.c, OpenMP over sites, FP64)"}
-        log(f"[bench] CPU oracle {dt:.2f}s on {cores} threads")
+               "label": "CPS-faithful (serial 4D hop, as wilson_dslash.C:280-477)",
+               "sample": f"1 MatPcDagMatPc application of the same workload in {t_ser:.2f}s "
+                         f"(oracle/cps_oracle.c, FP64; the 4D hop in one thread as CPS's "
+                         f"noarch build, the 5D terms and vector arithmetic on {cores} OpenMP "
+                         f"threads as CPS's *_omp.C)",
+               "ms_per_application": round(t_ser * 1e3, 1),
+               "gflops": round(flops_app / t_ser / 1e9, 3),
+               "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+               "omp_parallel_hop": {"label": "OMP-parallel restatement (4D hop over sites)",
+                                    "value": round(bytes_app / t_par / 1e9, 3), "unit": "GB/s",
+                                    "ms_per_application": round(t_par * 1e3, 1),
+                                    "cores": cores},
+               "dwf_8x8x8x8_ls8_solve": c1}
+        log(f"[bench] CPU {t_ser:.2f}s (serial hop) / {t_par:.2f}s (OpenMP hop) on {cores} "
+            f"threads; 8^3x8 solve CPU {c1['cpu_seconds']}s ({c1['cpu_iters']} it), GPU "
+            f"{c1['gpu_seconds']}s ({c1['gpu_iters']} it)")
 
     out = {
         "metric": METRIC,
@@ -331,12 +424,18 @@ def main():
         "gflops": round(gflops, 2),
         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
         "copy_kernel_gbs": round(copy_gbs, 1),
+        "bandwidth_probes_gbs": {"copy": round(bw_copy, 1), "read_only": round(bw_read, 1),
+                                 "three_reads_one_write": round(bw_r3w1, 1),
+                                 "peak": HBM_PEAK_GBS},
+        "hbm_frac_of_best_probe": round(gbs / max(bw_copy, bw_read, bw_r3w1), 4),
+        "parity_vs_oracle": parity,
         "kernels_ms": [round(float(x), 5) for x in ms_k],
         "sloppy_matpcdagmatpc": sloppy,
         "roofline": {"bound": "hbm", "achieved": round(k_gbs, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(k_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None if prof is None else
-                     prof.get("kernels", {}).get(KERNELS[wl["action"]][hop], {}).get("traffic_bytes"),
+                     "traffic": kprof.get("traffic_bytes"),
+                     "traffic_source": None if prof is None else prof.get("traffic_source"),
+                     "l2_fill_bytes": kprof.get("l2_fill_bytes"),
                      "kernel": KERNELS[wl["action"]][hop],
                      "algorithmic_bytes": bytes_k[hop]},
         "cpu_baseline": cpu,

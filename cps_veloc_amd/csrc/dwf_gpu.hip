@@ -339,15 +339,28 @@ int fetch_scalars(dwfgpu_s* h, int n, double* out) {
   return DWFGPU_OK;
 }
 
-int fetch_sum(dwfgpu_s* h, int nparts, double* out) {
-  hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(1024), 0, h->st, h->partial, nparts,
-                     h->dscal);
+// sum of nparts partials in `partial` through the device scalar `dscal`, to
+// the host (waits)
+int fetch_sum_to(dwfgpu_s* h, const double* partial, int nparts, double* dscal, double* out) {
+  hipLaunchKernelGGL(dwfk::k_reduce_partials, dim3(1), dim3(1024), 0, h->st, partial, nparts,
+                     dscal);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(h->hscal, h->dscal, sizeof(double), hipMemcpyDeviceToHost, h->st));
+  HIPCHK(hipMemcpyAsync(h->hscal, dscal, sizeof(double), hipMemcpyDeviceToHost, h->st));
   HIPCHK(hipStreamSynchronize(h->st));
   *out = h->hscal[0];
   return DWFGPU_OK;
 }
+int fetch_sum(dwfgpu_s* h, int nparts, double* out) {
+  return fetch_sum_to(h, h->partial, nparts, h->dscal, out);
+}
+
+// the temporaries of one MatPcDagMatPc: the opposite-parity field te, the
+// MatPc output t1, the per-work-group partial sums and one device scalar
+struct Scratch {
+  DVec te, t1;
+  double* partial;
+  double* dscal;
+};
 
 // ---------------------------------------------------------------------------
 // operator pieces for storage policy P
@@ -360,7 +373,8 @@ struct Ops {
 
   // DWF hop kernel: epi 0 -> out = D in ; epi 1 -> out = x0 + alpha D in
   static int dwf(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int cb_in, int dag,
-                 int epi, const DVec& x0, double alpha, bool norm) {
+                 int epi, const DVec& x0, double alpha, bool norm, double* partial = nullptr) {
+    if (!partial) partial = h->partial;
     const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const double two_a5 = 2.0 * h->a5inv, nm = -2.0 * mass * h->a5inv;
     const S* u = (const S*)F(h).U;
@@ -369,7 +383,7 @@ struct Ops {
     gm.rev = epi;  // the x0 + alpha D passes sweep backwards, as the Schur ones in mob()
 #define DWF_LAUNCH(DAG, EPI, NORM)                                                           \
   hipLaunchKernelGGL((dwfk::k_dwf_dslash<P, DAG, EPI, NORM>), grid, block, lds, h->st,       \
-                     fld<P>(out), cfld<P>(in), u, cfld<P>(x0), h->partial, gm, cb_in, two_a5, \
+                     fld<P>(out), cfld<P>(in), u, cfld<P>(x0), partial, gm, cb_in, two_a5,    \
                      nm, alpha, dwfk::CgEpi<P, P>{})
     if (epi == 0) {
       if (dag) DWF_LAUNCH(1, 0, false); else DWF_LAUNCH(0, 0, false);
@@ -385,7 +399,8 @@ struct Ops {
 
   // Mobius hop kernel; fl: dwfk::FL_* flags
   static int mob(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, const DVec& psi,
-                 int p_out, int dag, int epi, int fl) {
+                 int p_out, int dag, int epi, int fl, double* partial = nullptr) {
+    if (!partial) partial = h->partial;
     const dim3 grid(h->grid_hop), block(h->ns, h->ls);
     const MobArgs a = mob_args(h, mass, dag);
     // the Schur passes sweep backwards: they start on the sites the pass
@@ -407,7 +422,7 @@ struct Ops {
                                         col ? col_lds_bytes(h, sizeof(R)) : (size_t)0);
 #define MOB_LAUNCH(DAG, EPI, FL)                                                              \
   hipLaunchKernelGGL((dwfk::k_mobius<P, DAG, EPI, FL>), grid, block, lds, h->st, fld<P>(out), \
-                     cfld<P>(in), cfld<P>(psi), u, h->partial, gm, p_out, a, dwfk::CgEpi<P, P>{})
+                     cfld<P>(in), cfld<P>(psi), u, partial, gm, p_out, a, dwfk::CgEpi<P, P>{})
     using dwfk::FL_NORM;
     using dwfk::FL_POST_T;
     using dwfk::FL_PRE_T;
@@ -452,9 +467,9 @@ struct Ops {
     return mob(h, mass, te, in, DVec(), 0, dag, dwfk::EPI_M5INV, dag ? 0 : dwfk::FL_POST_T);
   }
   static int mob_second(dwfgpu_s* h, double mass, const DVec& out, const DVec& te,
-                        const DVec& in, int dag, bool norm) {
+                        const DVec& in, int dag, bool norm, double* partial = nullptr) {
     const int fl = (dag ? 0 : dwfk::FL_PRE_T) | (norm && !dag ? dwfk::FL_NORM : 0);
-    return mob(h, mass, out, te, in, 1, dag, dwfk::EPI_SCHUR, fl);
+    return mob(h, mass, out, te, in, 1, dag, dwfk::EPI_SCHUR, fl, partial);
   }
 
   static int m5inv(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int dag) {
@@ -471,28 +486,37 @@ struct Ops {
     return DWFGPU_OK;
   }
 
+  // the handle's own temporaries of this precision
+  static Scratch own(dwfgpu_s* h) { return Scratch{F(h).te, F(h).t1, h->partial, h->dscal}; }
+
   // MatPc / MatPcDag: out = M in (out on the CG parity).  norm -> partials.
-  static int matpc(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int dag,
-                   bool norm) {
-    const DVec& te = F(h).te;
+  static int matpc_w(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int dag,
+                     bool norm, const Scratch& w) {
     if (h->action == DWFGPU_DWF) {
       const double mk2 = -dwf_kappa(h) * dwf_kappa(h);
-      TRY(dwf(h, mass, te, in, 1, dag, 0, DVec(), 0.0, false));
-      TRY(dwf(h, mass, out, te, 0, dag, 1, in, mk2, norm));
+      TRY(dwf(h, mass, w.te, in, 1, dag, 0, DVec(), 0.0, false, w.partial));
+      TRY(dwf(h, mass, out, w.te, 0, dag, 1, in, mk2, norm, w.partial));
     } else {
-      TRY(mob_first(h, mass, te, in, dag));
-      TRY(mob_second(h, mass, out, te, in, dag, norm));
+      TRY(mob_first(h, mass, w.te, in, dag));
+      TRY(mob_second(h, mass, out, w.te, in, dag, norm, w.partial));
     }
     return DWFGPU_OK;
   }
+  static int matpc(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int dag,
+                   bool norm) {
+    return matpc_w(h, mass, out, in, dag, norm, own(h));
+  }
 
   // MatPcDagMatPc; *dot (if requested) = |MatPc in|^2 (dwf_mdagm.C:49-55)
-  static int mdagm(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, double* dot) {
-    const DVec& t1 = F(h).t1;
-    TRY(matpc(h, mass, t1, in, 0, dot != nullptr));
-    if (dot) TRY(fetch_sum(h, h->grid_hop, dot));
-    TRY(matpc(h, mass, out, t1, 1, false));
+  static int mdagm_w(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, double* dot,
+                     const Scratch& w) {
+    TRY(matpc_w(h, mass, w.t1, in, 0, dot != nullptr, w));
+    if (dot) TRY(fetch_sum_to(h, w.partial, h->grid_hop, w.dscal, dot));
+    TRY(matpc_w(h, mass, out, w.t1, 1, false, w));
     return DWFGPU_OK;
+  }
+  static int mdagm(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, double* dot) {
+    return mdagm_w(h, mass, out, in, dot, own(h));
   }
 
   // One CG step around MatPcDagMatPc (inv_cg.C:318-349) with the vector
@@ -2788,6 +2812,7 @@ extern "C" {
 
 const char* dwfgpu_last_error(void) { return g_err.c_str(); }
 
+
 int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, double m5,
                   double a5inv, double mob_b, double mob_c, const int bc[4], int precision,
                   dwfgpu_t* out) {
@@ -2857,6 +2882,7 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
     delete h;
     return DWFGPU_HIP_ERROR;
   }
+
   // sites per work-group of the hopping kernels; DWFGPU_NS overrides the
   // default.  Without an override, fewer sites per work-group are taken when
   // the LDS stage of a long fifth dimension would not fit.
@@ -3157,6 +3183,66 @@ int dwfgpu_time_copy(dwfgpu_t h, size_t bytes, int reps, double* gbs) {
   if (e1) (void)hipEventDestroy(e1);
   (void)hipFree(a);
   (void)hipFree(b);
+  return st;
+}
+
+int dwfgpu_time_bandwidth(dwfgpu_t h, size_t bytes, int reps, double* copy_gbs,
+                          double* read_gbs, double* r3w1_gbs) {
+  if (!h || reps < 1 || bytes < 16 || !copy_gbs || !read_gbs || !r3w1_gbs)
+    return DWFGPU_BAD_ARG;
+  typedef dwfk::u32x4_t W;
+  const size_t n16 = bytes / 16;
+  const int grids[3] = {2048, 4096, 8192};
+  W* buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  W* sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int st = DWFGPU_OK;
+  for (int k = 0; k < 4 && !st; ++k)
+    if (hipMalloc(&buf[k], n16 * 16) != hipSuccess ||
+        hipMemsetAsync(buf[k], k + 1, n16 * 16, h->st) != hipSuccess)
+      st = DWFGPU_HIP_ERROR;
+  if (!st && (hipMalloc(&sink, (size_t)grids[2] * 256 * 16) != hipSuccess ||
+              hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    st = DWFGPU_HIP_ERROR;
+  // bytes moved per launch: copy 2, read 1, three-read-one-write 4 buffers
+  const double moved[3] = {2.0, 1.0, 4.0};
+  double best[3] = {0.0, 0.0, 0.0};
+  for (int mode = 0; mode < 3 && !st; ++mode)
+    for (int gi = 0; gi < 3 && !st; ++gi) {
+      auto launch = [&]() {
+        const dim3 grid(grids[gi]), block(256);
+        if (mode == 0)
+          hipLaunchKernelGGL(dwfk::k_bw_probe<0>, grid, block, 0, h->st, buf[3], buf[0], buf[1],
+                             buf[2], n16, sink);
+        else if (mode == 1)
+          hipLaunchKernelGGL(dwfk::k_bw_probe<1>, grid, block, 0, h->st, buf[3], buf[0], buf[1],
+                             buf[2], n16, sink);
+        else
+          hipLaunchKernelGGL(dwfk::k_bw_probe<2>, grid, block, 0, h->st, buf[3], buf[0], buf[1],
+                             buf[2], n16, sink);
+      };
+      launch();  // untimed: first touch
+      if (hipEventRecord(e0, h->st) != hipSuccess) st = DWFGPU_HIP_ERROR;
+      for (int r = 0; r < reps && !st; ++r) launch();
+      float ms = 0;
+      if (!st && (hipGetLastError() != hipSuccess || hipEventRecord(e1, h->st) != hipSuccess ||
+                  hipEventSynchronize(e1) != hipSuccess ||
+                  hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+        st = DWFGPU_HIP_ERROR;
+      if (!st) best[mode] = std::max(best[mode], moved[mode] * n16 * 16 * reps / (ms * 1e-3) / 1e9);
+    }
+  if (!st) {
+    *copy_gbs = best[0];
+    *read_gbs = best[1];
+    *r3w1_gbs = best[2];
+  } else {
+    set_err("dwfgpu_time_bandwidth: HIP failure");
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  for (W* p : buf)
+    if (p) (void)hipFree(p);
+  if (sink) (void)hipFree(sink);
   return st;
 }
 

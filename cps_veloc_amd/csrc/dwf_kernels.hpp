@@ -1608,6 +1608,51 @@ k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {
     dst[k] = src[k];
 }
 
+// Bandwidth probes.  Each thread issues BW_UNROLL independent 16-byte loads
+// per stream before it stores anything (grid-stride, a plain tail loop for
+// the remainder); loads and stores are non-temporal.  MODE 0: d = a (copy);
+// MODE 1: read a only (the xor of a thread's words goes to sink[thread], a
+// grid-sized buffer, so the loads are kept); MODE 2: d = a ^ b ^ c (three
+// reads per write, the read:write mix of the hop kernels).
+constexpr int BW_UNROLL = 4;
+template <int MODE>
+__global__ void __launch_bounds__(256)
+k_bw_probe(u32x4_t* __restrict__ d, const u32x4_t* __restrict__ a,
+           const u32x4_t* __restrict__ b, const u32x4_t* __restrict__ c, size_t n16,
+           u32x4_t* __restrict__ sink) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  u32x4_t acc = {0u, 0u, 0u, 0u};
+  size_t k = tid;
+  for (; k + (BW_UNROLL - 1) * stride < n16; k += BW_UNROLL * stride) {
+    u32x4_t v[BW_UNROLL];
+#pragma unroll
+    for (int j = 0; j < BW_UNROLL; ++j) v[j] = __builtin_nontemporal_load(a + k + j * stride);
+    if constexpr (MODE == 2) {
+#pragma unroll
+      for (int j = 0; j < BW_UNROLL; ++j)
+        v[j] ^= __builtin_nontemporal_load(b + k + j * stride) ^
+                __builtin_nontemporal_load(c + k + j * stride);
+    }
+    if constexpr (MODE == 1) {
+#pragma unroll
+      for (int j = 0; j < BW_UNROLL; ++j) acc ^= v[j];
+    } else {
+#pragma unroll
+      for (int j = 0; j < BW_UNROLL; ++j) __builtin_nontemporal_store(v[j], d + k + j * stride);
+    }
+  }
+  for (; k < n16; k += stride) {
+    u32x4_t v = __builtin_nontemporal_load(a + k);
+    if constexpr (MODE == 2) v ^= b[k] ^ c[k];
+    if constexpr (MODE == 1)
+      acc ^= v;
+    else
+      d[k] = v;
+  }
+  if constexpr (MODE == 1) sink[tid] = acc;
+}
+
 // deterministic final sum of per-block partials (one work-group of 1024
 // threads, four independent accumulators per thread, fixed order)
 __global__ void __launch_bounds__(1024)

@@ -46,6 +46,7 @@ EXPORTS = (
     "dwfgpu_ptc_get_weights", "dwfgpu_ptc_apply", "dwfgpu_ptc_loss_grad", "dwfgpu_ptc_train",
     "dwfgpu_invcg_ptc_dev", "dwfgpu_minvcg", "dwfgpu_minvcg_dev",
     "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
+    "dwfgpu_time_bandwidth",
     "dwfgpu_fermion_from_canonical", "dwfgpu_fermion_to_canonical", "dwfgpu_four2five",
     "dwfgpu_five2four", "dwfgpu_matinv_dev", "dwfgpu_nersc_read", "dwfgpu_plaquette",
     "dwfgpu_inveigcg_dev", "dwfgpu_eigcg_create", "dwfgpu_eigcg_solve", "dwfgpu_eigcg_def_len",
@@ -174,6 +175,8 @@ def lib():
         L.dwfgpu_matpcdagmatpc_cost.argtypes = [H, C.c_int, C.POINTER(C.c_double),
                                                 C.POINTER(C.c_double)]
         L.dwfgpu_time_copy.argtypes = [H, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.dwfgpu_time_bandwidth.argtypes = [H, C.c_size_t, C.c_int, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.dwfgpu_fermion_from_canonical.argtypes = [H, H, H, _dp]
         L.dwfgpu_fermion_to_canonical.argtypes = [H, _dp, H, H]
         L.dwfgpu_four2five.argtypes = [H, H, H, _dp, C.c_int, C.c_int]
@@ -508,6 +511,13 @@ class _DiracOpGpu:
         g = C.c_double()
         _chk(lib().dwfgpu_time_copy(self.h, int(nbytes), reps, C.byref(g)), "time_copy")
         return g.value
+
+    def time_bandwidth(self, nbytes, reps=10):
+        """(copy, read-only, three-read-one-write) GB/s of the device's probes."""
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _chk(lib().dwfgpu_time_bandwidth(self.h, int(nbytes), reps, C.byref(a), C.byref(b),
+                                         C.byref(c)), "time_bandwidth")
+        return a.value, b.value, c.value
 
     def cost(self, precision=None):
         b, f = C.c_double(), C.c_double()

@@ -265,6 +265,15 @@ int dwfgpu_matpcdagmatpc_cost(dwfgpu_t h, int precision, double *bytes, double *
  * down to 16) between two fresh device buffers `reps` times; *gbs receives
  * (read + write bytes) / mean kernel time.                                   */
 int dwfgpu_time_copy(dwfgpu_t h, size_t bytes, int reps, double *gbs);
+/* Memory-bandwidth ceilings of this device, beside which the operator's
+ * GB/s are read: the best over three grid sizes of a copy (*copy_gbs, read +
+ * write bytes), a read-only pass (*read_gbs) and a pass reading three buffers
+ * and writing one (*r3w1_gbs, the read:write mix of the hop kernels), each
+ * over `bytes` per buffer with 4 independent 16-byte non-temporal loads in
+ * flight per thread.  Allocates and frees 4 buffers of `bytes`.  No
+ * counterpart in CPS (measurement only).                                     */
+int dwfgpu_time_bandwidth(dwfgpu_t h, size_t bytes, int reps, double *copy_gbs,
+                          double *read_gbs, double *r3w1_gbs);
 
 /* ---- (f2) host formats of whole fermion fields and gauge files --------- */
 /* Full 5D fermion in CANONICAL order, FdwfBase::Fconvert(CANONICAL <->

@@ -308,13 +308,19 @@ static void umu_times(double *into, const double *u, const double *psi, int mu,
 /* chi (sites of parity 1-cb) = sum_mu (1 - s gamma_mu) U_mu(x) psi(x+mu)
  *                             + (1 + s gamma_mu) U_mu(x-mu)^dag psi(x-mu),
  * s = +1 (dag=0) or -1 (dag=1).  Periodic wrap; boundary signs live in U.   */
+/* CPS's noarch 4D hop walks the sites in one thread (wilson_dslash.C:280-477);
+ * the restatement spreads them over OpenMP threads unless this is set.  Each
+ * site is written once from its own inputs, so both give the same bits.    */
+static int g_serial_hop = 0;
+void orc_set_serial_hop(int on) { g_serial_hop = on != 0; }
+
 void orc_wilson_dslash(double *chi_p, const double *u_p, const double *psi_p, int cb,
                        int dag, const int L[4]) {
   const int lx = L[0], ly = L[1], lz = L[2], lt = L[3];
   const int vol = vol4(L) / 2; /* checkerboard volume, wilson_p->vol[0] */
   const int sdag = dag ? -1 : 1;
   const int cbn = cb ? 0 : 1;
-#pragma omp parallel for collapse(2) schedule(static)
+#pragma omp parallel for collapse(2) schedule(static) if (!g_serial_hop)
   for (int x = 0; x < lx; ++x)
     for (int y = 0; y < ly; ++y)
       for (int z = 0; z < lz; ++z)

@@ -99,6 +99,10 @@ void orc_five2four(double *four, const double *five, int s_u, int s_l, const orc
 /* ---- a1: Wilson 4D hopping term ------------------------------------- */
 void orc_wilson_dslash(double *chi, const double *u, const double *psi,
                        int cb, int dag, const int L[4]);
+/* 1: the 4D hop walks its sites in one thread, as CPS's noarch
+ * wilson_dslash.C:280-477 does (the timing of the CPS-faithful CPU baseline);
+ * 0 (default): OpenMP over sites.  Same bits either way.                  */
+void orc_set_serial_hop(int on);
 
 /* ---- a2..a4: DWF ----------------------------------------------------- */
 void orc_dwf_dslash_5_plus(double *out, const double *in, double mass, int dag,

@@ -67,10 +67,18 @@ def lib():
         L.orc_fermion5d_full_to_canonical.argtypes = [_dp, _dp, P]
         L.orc_four2five.argtypes = [_dp, _dp, C.c_int, C.c_int, P]
         L.orc_five2four.argtypes = [_dp, _dp, C.c_int, C.c_int, P]
+        L.orc_set_serial_hop.argtypes = [C.c_int]
         L.orc_norm_sq.restype = C.c_double
         L.orc_norm_sq.argtypes = [_dp, C.c_size_t]
         _lib = L
     return _lib
+
+
+def set_serial_hop(on: bool) -> None:
+    """Run the 4D hop in one thread, as CPS's noarch wilson_dslash.C:280-477
+    does (the rest keeps its OpenMP loops, as dwf_dslash_5_plus_omp.C and
+    vector_util.C do); results are bit-identical either way."""
+    lib().orc_set_serial_hop(1 if on else 0)
 
 
 class Oracle:

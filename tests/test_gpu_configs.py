@@ -54,6 +54,21 @@ def site_rel(got, want):
     return float(np.max(np.linalg.norm(g - w, axis=1) / np.maximum(nw, 1e-300)))
 
 
+def site_rel_sparse(got, want, floor=1e-6):
+    """(per-site relative error over the sites whose norm is at least `floor`
+    of the largest, largest site error over the largest site norm).  Point
+    sources leave sites whose exact value is zero by cancellation; both sides
+    hold rounding residue there (down to 1e-21 of the largest site), where a
+    per-site relative error means nothing, so those sites are bounded by the
+    second number."""
+    g = np.asarray(got).reshape(-1, 24)
+    w = np.asarray(want).reshape(-1, 24)
+    nw = np.linalg.norm(w, axis=1)
+    ne = np.linalg.norm(g - w, axis=1)
+    big = nw >= floor * nw.max()
+    return float(np.max(ne[big] / nw[big])), float(ne.max() / nw.max())
+
+
 def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
@@ -172,12 +187,19 @@ def test_dwf_16x16x16x32_ls16_mrhs24_vs_oracle():
     srcs = _mrhs_sources(o, dims, ls, masses, kappa)
     vin, vout = g.multi_vector(24, srcs), g.multi_vector(24)
     g.MatPcDagMatPc_mrhs_dev(vout, vin, masses, "f64")
-    worst = 0.0
+    worst, worst_abs = 0.0, 0.0
+    single = np.zeros(g.n)
     for j in range(24):
+        got = vout.download(j)
         want, _ = o.matpcdagmatpc(srcs[j], masses[j])
-        worst = max(worst, site_rel(vout.download(j), want))
-    say(f"  batched FP64 operator, 24 RHS: max site rel err {worst:.2e}")
-    assert worst < 1e-13
+        r1, r2 = site_rel_sparse(got, want)
+        worst, worst_abs = max(worst, r1), max(worst_abs, r2)
+        if j in (0, 5, 12, 23):  # the single-RHS operator: same arithmetic per site
+            g.MatPcDagMatPc(single, srcs[j], masses[j])
+            assert np.array_equal(got, single), j
+    say(f"  batched FP64 operator, 24 RHS: max site rel err {worst:.2e} (sites above 1e-6 of "
+        f"the largest), max site err / largest site {worst_abs:.2e}")
+    assert worst < 1e-13 and worst_abs < 1e-14
     arg = CgArg(0.0, 5000, 1e-10)
     sol = g.multi_vector(24)
     it, tr, st, nrel = g.InvCgMrhs_dev(sol, vin, arg, masses, "f32", 0.1)

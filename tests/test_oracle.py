@@ -61,6 +61,23 @@ def test_boundary_conditions_vs_textbook(bc):
     assert rel(cplx(out).reshape(-1, 4, 3), full.reshape(-1, 4, 3)[g[0]]) < 1e-13
 
 
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
+def test_serial_hop_same_bits(action):
+    """The CPS-faithful timing variant (4D hop in one thread, as
+    wilson_dslash.C:280-477) computes exactly what the OpenMP one does."""
+    from oracle import oracle as orc
+    dims, ls = (4, 4, 4, 8), 4
+    _, o = setup(dims, ls, action)
+    x = lat.to_reals(lat.gaussian_spinor((ls, int(np.prod(dims)) // 2), 8))
+    want, dwant = o.matpcdagmatpc(x, 0.04)
+    orc.set_serial_hop(True)
+    try:
+        got, dgot = o.matpcdagmatpc(x, 0.04)
+    finally:
+        orc.set_serial_hop(False)
+    assert np.array_equal(got, want) and dgot == dwant
+
+
 def test_free_field_plane_wave():
     """U = 1: a plane wave e^{ipx} is an eigenvector of the hopping term with
     multiplier sum_mu [(1-g_mu) e^{ip_mu} + (1+g_mu) e^{-ip_mu}] (closed form)."""
