@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -79,6 +80,7 @@ struct PrecFields {
 
 struct dwfgpu_s {
   int action = 0, prec = 0;  // prec: storage precision of the API vectors
+  int dev = 0;               // HIP device the handle was created on
   int dims[4] = {0, 0, 0, 0};
   int bc[4] = {0, 0, 0, 0};
   int ls = 0;
@@ -2812,6 +2814,73 @@ extern "C" {
 
 const char* dwfgpu_last_error(void) { return g_err.c_str(); }
 
+// ---------------------------------------------------------------------------
+// allocation-free MatPcDagMatPc (dwfgpu_matpcdagmatpc_dev_noalloc): its
+// temporaries live in one static device array, which every GPU the library
+// runs on holds from module load to the end of the process.  In host code
+// `scratch` only names the symbol; its device address (one per GPU) comes from
+// hipGetSymbolAddress, once per device, when the first handle is created there.
+// ---------------------------------------------------------------------------
+extern "C++" {
+__device__ __align__(256) unsigned char scratch[DWFGPU_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_dev[kMaxDevices] = {};
+
+// with `dev` current: look up this device's copy of `scratch`
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_err("scratch_setup: device %d beyond the %d the library tracks", dev, kMaxDevices);
+    return DWFGPU_BAD_ARG;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev]) return DWFGPU_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    set_err("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    return DWFGPU_HIP_ERROR;
+  }
+  g_scratch_dev[dev] = (unsigned char*)ptr;
+  return DWFGPU_OK;
+}
+
+size_t round256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// carve te, t1 (with their scales in the 16-bit formats), grid_hop partials
+// and one device scalar out of base (nullptr: only count); returns the bytes
+template <class P>
+size_t carve_scratch(const dwfgpu_s* h, unsigned char* base, Scratch* w) {
+  const size_t vb = round256((size_t)(h->n / 2) * sizeof(typename P::S));
+  const size_t sb = P::kScaled ? round256((size_t)h->g.vh * h->ls * sizeof(float)) : 0;
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> unsigned char* {
+    unsigned char* p = base ? base + off : nullptr;
+    off += bytes;
+    return p;
+  };
+  DVec te, t1;
+  te.d = take(vb);
+  te.sc = P::kScaled ? (float*)take(sb) : nullptr;
+  t1.d = take(vb);
+  t1.sc = P::kScaled ? (float*)take(sb) : nullptr;
+  double* partial = (double*)take(round256((size_t)h->grid_hop * sizeof(double)));
+  double* dscal = (double*)take(256);
+  if (w) *w = Scratch{te, t1, partial, dscal};
+  return off;
+}
+size_t scratch_need(const dwfgpu_s* h) {
+  switch (h->prec) {
+    case DWFGPU_F64: return carve_scratch<P64>(h, nullptr, nullptr);
+    case DWFGPU_F32: return carve_scratch<P32>(h, nullptr, nullptr);
+    case DWFGPU_BF16: return carve_scratch<PBF16>(h, nullptr, nullptr);
+    default: return carve_scratch<PI16>(h, nullptr, nullptr);
+  }
+}
+}  // namespace
+}  // extern "C++"
 
 int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, double m5,
                   double a5inv, double mob_b, double mob_c, const int bc[4], int precision,
@@ -2882,7 +2951,11 @@ int dwfgpu_create(const double* gauge, const int dims[4], int ls, int action, do
     delete h;
     return DWFGPU_HIP_ERROR;
   }
-
+  h->dev = dev;
+  if (int rs = scratch_setup(dev)) {
+    delete h;
+    return rs;
+  }
   // sites per work-group of the hopping kernels; DWFGPU_NS overrides the
   // default.  Without an override, fewer sites per work-group are taken when
   // the LDS stage of a long fifth dimension would not fit.
@@ -3084,6 +3157,53 @@ int dwfgpu_matpcdagmatpc_dev(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_v
                              double* dot_prd) {
   if (!h || !out || !in || out->h != h || in->h != h) return DWFGPU_BAD_ARG;
   TRY(DISPATCH(h, mdagm(h, mass, out->v, in->v, dot_prd)));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
+}
+
+int dwfgpu_matpcdagmatpc_dev_noalloc(dwfgpu_t h, double mass, dwfgpu_vec_t out,
+                                     dwfgpu_vec_t in, double* dot_prd) {
+  if (!h || !out || !in || out->h != h || in->h != h) return DWFGPU_BAD_ARG;
+  const size_t need = scratch_need(h);
+  if (need > DWFGPU_NOALLOC_SCRATCH_BYTES) {
+    set_err("dwfgpu_matpcdagmatpc_dev_noalloc: this lattice needs %zu bytes of scratch, the "
+            "static array holds %zu (use dwfgpu_matpcdagmatpc_dev)",
+            need, (size_t)DWFGPU_NOALLOC_SCRATCH_BYTES);
+    return DWFGPU_BAD_ARG;
+  }
+  unsigned char* base = g_scratch_dev[h->dev];
+  if (!base) {
+    set_err("dwfgpu_matpcdagmatpc_dev_noalloc: no scratch address for device %d", h->dev);
+    return DWFGPU_HIP_ERROR;
+  }
+  // every kernel writes all of te, t1 and the partials it later reads, so
+  // nothing left by an earlier call is consumed and nothing needs clearing
+  switch (h->prec) {
+    case DWFGPU_F64: {
+      Scratch w;
+      carve_scratch<P64>(h, base, &w);
+      TRY(Ops<P64>::mdagm_w(h, mass, out->v, in->v, dot_prd, w));
+      break;
+    }
+    case DWFGPU_F32: {
+      Scratch w;
+      carve_scratch<P32>(h, base, &w);
+      TRY(Ops<P32>::mdagm_w(h, mass, out->v, in->v, dot_prd, w));
+      break;
+    }
+    case DWFGPU_BF16: {
+      Scratch w;
+      carve_scratch<PBF16>(h, base, &w);
+      TRY(Ops<PBF16>::mdagm_w(h, mass, out->v, in->v, dot_prd, w));
+      break;
+    }
+    default: {
+      Scratch w;
+      carve_scratch<PI16>(h, base, &w);
+      TRY(Ops<PI16>::mdagm_w(h, mass, out->v, in->v, dot_prd, w));
+      break;
+    }
+  }
   HIPCHK(hipStreamSynchronize(h->st));
   return DWFGPU_OK;
 }

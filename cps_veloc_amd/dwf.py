@@ -25,6 +25,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("CPS_DWF_GPU_LIB", os.path.join(_HERE, "libcps_dwf_gpu.so"))
 
 OK, BREAKDOWN, MAXITER, BAD_ARG, HIP_ERROR, IO_ERROR, CHECKSUM_ERROR = 0, 1, 2, 3, 4, 5, 6
+# bytes of the static device array behind dwfgpu_matpcdagmatpc_dev_noalloc
+NOALLOC_SCRATCH_BYTES = 256 << 20
 DWF, MOBIUS = 0, 1
 F64, F32, BF16, I16 = 0, 1, 2, 3
 _PREC = {"f64": F64, "f32": F32, "bf16": BF16, "i16": I16}
@@ -38,6 +40,7 @@ EXPORTS = (
     "dwfgpu_matpcdagmatpc", "dwfgpu_matpc", "dwfgpu_dslash", "dwfgpu_invcg",
     "dwfgpu_invcg_shift", "dwfgpu_matinv", "dwfgpu_vec_create", "dwfgpu_vec_destroy",
     "dwfgpu_vec_upload", "dwfgpu_vec_download", "dwfgpu_matpcdagmatpc_dev", "dwfgpu_invcg_dev",
+    "dwfgpu_matpcdagmatpc_dev_noalloc",
     "dwfgpu_invcg_shift_dev",
     "dwfgpu_invcg_mixed", "dwfgpu_invcg_mixed_dev",
     "dwfgpu_mvec_create", "dwfgpu_mvec_destroy", "dwfgpu_mvec_upload", "dwfgpu_mvec_download",
@@ -135,6 +138,8 @@ def lib():
         L.dwfgpu_vec_upload.argtypes = [H, _dp]
         L.dwfgpu_vec_download.argtypes = [H, _dp]
         L.dwfgpu_matpcdagmatpc_dev.argtypes = [H, C.c_double, H, H, C.POINTER(C.c_double)]
+        L.dwfgpu_matpcdagmatpc_dev_noalloc.argtypes = [H, C.c_double, H, H,
+                                                       C.POINTER(C.c_double)]
         L.dwfgpu_invcg_dev.argtypes = [H, C.POINTER(CgArgC), H, H, C.POINTER(C.c_double),
                                        C.POINTER(C.c_int)]
         L.dwfgpu_invcg_mixed.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, _dp, _dp,
@@ -365,6 +370,14 @@ class _DiracOpGpu:
         d = C.c_double()
         _chk(lib().dwfgpu_matpcdagmatpc_dev(self.h, mass, out.h, inp.h, C.byref(d)),
              "MatPcDagMatPc_dev")
+        return d.value
+
+    def MatPcDagMatPc_dev_noalloc(self, out: DeviceVector, inp: DeviceVector, mass):
+        """MatPcDagMatPc_dev with its temporaries in the library's static device
+        array (no device allocation; lattices up to NOALLOC_SCRATCH_BYTES)."""
+        d = C.c_double()
+        _chk(lib().dwfgpu_matpcdagmatpc_dev_noalloc(self.h, mass, out.h, inp.h, C.byref(d)),
+             "MatPcDagMatPc_dev_noalloc")
         return d.value
 
     def InvCg_dev(self, sol: DeviceVector, src: DeviceVector, cg_arg: CgArg):

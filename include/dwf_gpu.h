@@ -122,6 +122,19 @@ int dwfgpu_vec_upload(dwfgpu_vec_t v, const double *host_cps_order);
 int dwfgpu_vec_download(dwfgpu_vec_t v, double *host_cps_order);
 int dwfgpu_matpcdagmatpc_dev(dwfgpu_t h, double mass, dwfgpu_vec_t out, dwfgpu_vec_t in,
                              double *dot_prd);
+/* No counterpart in CPS.  dwfgpu_matpcdagmatpc_dev for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): the same kernels in the same order, with
+ * their temporaries (te and t1 half vectors in the handle's precision, the
+ * per-work-group partial sums, one device scalar) in one static device array
+ * of DWFGPU_NOALLOC_SCRATCH_BYTES per GPU, held from module load to process
+ * end.  It calls no hipMalloc / hipFree.  A lattice whose temporaries do not
+ * fit (2 FP64 half vectors: up to about 690k CG sites, e.g. 16^3x16 Ls 10)
+ * gets DWFGPU_BAD_ARG.  Every call on one GPU shares the array: calls that
+ * overlap on one GPU (two host threads, two handles' streams) race.        */
+#define DWFGPU_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int dwfgpu_matpcdagmatpc_dev_noalloc(dwfgpu_t h, double mass, dwfgpu_vec_t out,
+                                     dwfgpu_vec_t in, double *dot_prd);
 int dwfgpu_invcg_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol,
                      dwfgpu_vec_t src, double *true_res, int *iters);
 int dwfgpu_invcg_shift_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, dwfgpu_vec_t sol,
