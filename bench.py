@@ -43,7 +43,12 @@ WORKLOADS = {
     # s = Ls-1) and 12 complex Z2 noise sources, masses 0.005 / 0.01 / 0.04
     "dwf_16x16x16x32_ls16_mrhs24": dict(action="dwf", dims=(16, 16, 16, 32), ls=16, mass=0.01,
                                         eps=0.3, nrhs=24, masses=(0.005, 0.01, 0.04)),
+    "mobius_16x16x16x32_ls16_mrhs24": dict(action="mobius", dims=(16, 16, 16, 32), ls=16,
+                                           mass=0.01, eps=0.3, b=1.5, c=0.5, nrhs=24,
+                                           masses=(0.005, 0.01, 0.04)),
 }
+for _name, _wl in WORKLOADS.items():
+    _wl["name"] = _name
 
 
 def log(*a):
@@ -140,90 +145,111 @@ def c1_solves():
             "gpu_status": gst, "rel_err_gpu_vs_cpu_solution": err}
 
 
-def mrhs_sources(op, wl, kappa, seed=99):
-    """CG sources of the batched workload: full 5D sources (point or Z2),
-    prepared as DiracOpDwf::MatInv does (d_op_dwf.C:357-415):
-    b = MatPcDag(eta_odd + kappa Dslash(eta_even))."""
+def mrhs_full_sources(wl, seed=99):
+    """The full 5D sources of the batched workload in CPS's DWF order
+    [5D-odd | 5D-even]: 12 spin-colour point sources at the origin (spins 0, 1
+    on s = 0 and spins 2, 3 on s = Ls-1, as Ffour2five(src, 0, Ls-1) places
+    them, QPropW.C:1204-1205) and complex Z2 noise on every site; masses
+    cycle through wl["masses"]."""
     rng = np.random.default_rng(seed)
-    n, ls = op.n, wl["ls"]
+    ls = wl["ls"]
     vh = int(np.prod(wl["dims"])) // 2
-    srcs, masses = [], []
+    firsts, seconds, masses = [], [], []
     for j in range(wl["nrhs"]):
-        mass = wl["masses"][j % len(wl["masses"])]
         ev = np.zeros((ls, vh, 4, 3, 2))
         od = np.zeros((ls, vh, 4, 3, 2))
-        if j < 12:                   # spin-colour point source at the origin
+        if j < 12:
             spin, col = divmod(j, 3)
             if spin < 2:
-                ev[0, 0, spin, col, 0] = 1.0          # P+ -> s = 0 (5D-even half)
+                ev[0, 0, spin, col, 0] = 1.0
             else:
-                od[ls - 1, 0, spin, col, 0] = 1.0     # P- -> s = Ls-1 (5D-odd half)
-        else:                        # complex Z2 noise on every site
+                od[ls - 1, 0, spin, col, 0] = 1.0
+        else:
             ev = rng.choice([-1.0, 1.0], size=ev.shape) / np.sqrt(2.0)
             od = rng.choice([-1.0, 1.0], size=od.shape) / np.sqrt(2.0)
-        t = np.zeros(n)
-        op.Dslash(t, ev.reshape(-1), mass, 0, 0)
-        t = od.reshape(-1) + kappa * t
-        b = np.zeros(n)
-        op.MatPcDag(b, t, mass)
-        srcs.append(b)
-        masses.append(mass)
-    return srcs, np.array(masses)
+        firsts.append(od.reshape(-1))
+        seconds.append(ev.reshape(-1))
+        masses.append(wl["masses"][j % len(wl["masses"])])
+    return firsts, seconds, np.array(masses)
 
 
-def bench_mrhs(args, wl):
-    """Batched solve of all RHS vs the same RHS one at a time."""
+def run_batched(wl, sloppy, delta, max_iter):
+    """DiracOp*::MatInv over all RHS of the workload: one batched solve
+    (dwfgpu_matinv_mrhs_dev: sources prepared and solutions completed on the
+    device, one graph-replayed CG for all RHS) against the same entry point
+    called one RHS at a time.  Returns the record of the bench line."""
     from cps_veloc_amd import lattice as lat
-    from cps_veloc_amd.dwf import CgArg, DiracOpDwfGpu
+    from cps_veloc_amd.dwf import CgArg, DiracOpDwfGpu, DiracOpMobiusGpu
     dims, ls = wl["dims"], wl["ls"]
-    log(f"[bench] workload {args.workload}: gauge and {wl['nrhs']} sources")
     U = lat.random_gauge(dims, wl["eps"], 1234)
-    op = DiracOpDwfGpu(U, dims, ls, m5=1.8)
-    kappa = 1.0 / (2.0 * (4.0 + 1.0 - 1.8))
-    srcs, masses = mrhs_sources(op, wl, kappa)
-    nrhs = len(srcs)
-    vsrc, vsol = op.multi_vector(nrhs, srcs), op.multi_vector(nrhs)
-    arg = CgArg(0.0, args.max_iter, 1e-10)
-    # warm-up: the autotuner and graph set-up of the batched path
-    op.InvCgMrhs_dev(vsol, vsrc, CgArg(0.0, 3, 1e-10), masses, args.mrhs_sloppy, args.delta)
+    if wl["action"] == "dwf":
+        op = DiracOpDwfGpu(U, dims, ls, m5=1.8)
+    else:
+        op = DiracOpMobiusGpu(U, dims, ls, m5=1.8, b=wl["b"], c=wl["c"])
+    firsts, seconds, masses = mrhs_full_sources(wl)
+    nrhs = len(masses)
+    inf, ins = op.multi_vector(nrhs, firsts), op.multi_vector(nrhs, seconds)
+    outf, outs = op.multi_vector(nrhs), op.multi_vector(nrhs)
+    arg = CgArg(0.0, max_iter, 1e-10)
+    o1 = [op.multi_vector(1) for _ in range(4)]  # out_first, out_second, in_first, in_second
+    # warm-up: autotuner, graph set-up and first-use allocations of both calls
+    op.MatInvMrhs_dev(outf, outs, inf, ins, CgArg(0.0, 3, 1e-10), masses, sloppy, delta)
+    o1[2].upload(0, firsts[0])
+    o1[3].upload(0, seconds[0])
+    op.MatInvMrhs_dev(o1[0], o1[1], o1[2], o1[3], CgArg(0.0, 3, 1e-10), masses[:1], sloppy,
+                      delta)
     for j in range(nrhs):
-        vsol.upload(j, np.zeros(op.n))
-    log("[bench] batched solve")
+        outf.upload(j, np.zeros(op.n))
+        outs.upload(j, np.zeros(op.n))
+    log(f"[bench] batched MatInv of {nrhs} RHS")
     t0 = time.perf_counter()
-    it, tr, st, nrel = op.InvCgMrhs_dev(vsol, vsrc, arg, masses, args.mrhs_sloppy, args.delta)
+    it, tr, st, nrel = op.MatInvMrhs_dev(outf, outs, inf, ins, arg, masses, sloppy, delta)
     t_batch = time.perf_counter() - t0
     ns, nr = op.mrhs_shape()
     log(f"[bench] batched: {t_batch:.2f}s, iters {it.tolist()}, shape ns={ns} nr={nr}")
-    xb = [vsol.download(j) for j in range(nrhs)]
-    # the same RHS one at a time with the single-RHS mixed solver
-    v1, s1 = op.vector(), op.vector()
     t_seq, it_seq, err = 0.0, [], []
     for j in range(nrhs):
-        s1.upload(srcs[j])
-        v1.upload(np.zeros(op.n))
-        a = CgArg(masses[j], args.max_iter, 1e-10)
+        o1[2].upload(0, firsts[j])
+        o1[3].upload(0, seconds[j])
+        o1[0].upload(0, np.zeros(op.n))
+        o1[1].upload(0, np.zeros(op.n))
         t0 = time.perf_counter()
-        i1, tr1, st1, _ = op.InvCgMixed_dev(v1, s1, a, args.mrhs_sloppy, args.delta)
+        i1, _, _, _ = op.MatInvMrhs_dev(o1[0], o1[1], o1[2], o1[3], arg, masses[j:j + 1],
+                                        sloppy, delta)
         t_seq += time.perf_counter() - t0
-        it_seq.append(i1)
-        x1 = v1.download()
-        err.append(float(np.linalg.norm(xb[j] - x1) / np.linalg.norm(x1)))
+        it_seq.append(int(i1[0]))
+        x1 = np.concatenate([o1[0].download(0), o1[1].download(0)])
+        xb = np.concatenate([outf.download(j), outs.download(j)])
+        err.append(float(np.linalg.norm(xb - x1) / np.linalg.norm(x1)))
     log(f"[bench] one at a time: {t_seq:.2f}s, iters {it_seq}")
+    for v in [inf, ins, outf, outs] + o1:
+        v.free()
+    op.close()
+    return {"workload": wl["name"], "action": wl["action"], "lattice": list(dims), "ls": ls,
+            "nrhs": nrhs, "masses": masses.tolist(), "stop_rsd": 1e-10, "sloppy": sloppy,
+            "delta": delta, "shape": {"ns": ns, "nr": nr},
+            "batched": {"seconds": round(t_batch, 3), "iters": it.tolist(),
+                        "true_res_max": float(np.max(tr)), "status": st.tolist(),
+                        "reliable_updates": nrel.tolist()},
+            "one_at_a_time": {"seconds": round(t_seq, 3), "iters": it_seq},
+            "rhs_per_second_batched": round(nrhs / t_batch, 3),
+            "speedup_batched_vs_one_at_a_time": round(t_seq / t_batch, 3),
+            "max_rel_diff_batched_vs_single": max(err)}
+
+
+def bench_mrhs(args, wl):
+    """Bench line of the batched workload alone (--workload ..._mrhs24)."""
+    rec = run_batched(wl, args.mrhs_sloppy, args.delta, args.max_iter)
     out = {
-        "metric": "batched 24-RHS DWF solve to 1e-10: right-hand sides solved per second",
-        "value": round(nrhs / t_batch, 4), "unit": "RHS/s", "n_gpus": 1, "steps": 1,
-        "warmup": 1, "ms_per_step": round(t_batch * 1e3, 2), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": args.mrhs_sloppy + "+f64",
-        "data": "synthetic",
-        "config": {"workload": args.workload, "lattice": list(dims), "ls": ls,
-                   "masses": masses.tolist(), "stop_rsd": 1e-10, "delta": args.delta,
-                   "sloppy": args.mrhs_sloppy, "shape": {"ns": ns, "nr": nr}},
-        "batched": {"seconds": round(t_batch, 3), "iters": it.tolist(),
-                    "true_res": tr.tolist(), "status": st.tolist(),
-                    "reliable_updates": nrel.tolist()},
-        "one_at_a_time": {"seconds": round(t_seq, 3), "iters": it_seq},
-        "speedup_batched_vs_one_at_a_time": round(t_seq / t_batch, 3),
-        "max_rel_diff_batched_vs_single": max(err),
+        "metric": "batched 24-RHS MatInv to 1e-10: right-hand sides solved per second",
+        "value": rec["rhs_per_second_batched"], "unit": "RHS/s", "n_gpus": 1, "steps": 1,
+        "warmup": 1, "ms_per_step": round(rec["batched"]["seconds"] * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.mrhs_sloppy + "+f64", "data": "synthetic",
+        "config": {"workload": args.workload, "lattice": rec["lattice"], "ls": rec["ls"],
+                   "masses": rec["masses"], "stop_rsd": 1e-10, "delta": args.delta,
+                   "sloppy": args.mrhs_sloppy},
+        "batched_matinv": rec,
     }
     print(json.dumps(out), flush=True)
 
@@ -243,6 +269,8 @@ def main():
     ap.add_argument("--delta", type=float, default=0.1,
                     help="reliable update when |r| drops by this factor")
     ap.add_argument("--mrhs-sloppy", default="f32", choices=["f32", "bf16", "i16"])
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the 24-RHS batched MatInv leg (DWF 16^3x32, Ls 16)")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training the network preconditioner and its solve")
     ap.add_argument("--learned-sloppy", default="i16", choices=["f32", "bf16", "i16"])
@@ -375,6 +403,12 @@ def main():
                              "speedup_vs_plain_fp64": round(t_plain / best[1], 3)}
         vsol.free()
 
+    batched = None
+    if not args.no_batched:
+        log("[bench] 24-RHS batched MatInv, DWF 16^3x32 Ls 16")
+        batched = run_batched(WORKLOADS["dwf_16x16x16x32_ls16_mrhs24"], args.mrhs_sloppy,
+                              args.delta, args.max_iter)
+
     cpu, parity = None, None
     if not args.no_cpu:
         log("[bench] CPU baseline: one MatPcDagMatPc application, serial 4D hop and OpenMP hop")
@@ -440,6 +474,7 @@ def main():
                      "algorithmic_bytes": bytes_k[hop]},
         "cpu_baseline": cpu,
         "solve": solve,
+        "batched_24rhs": batched,
     }
     print(json.dumps(out), flush=True)
 

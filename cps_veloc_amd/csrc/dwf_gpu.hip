@@ -228,6 +228,7 @@ M5Side m5_side(const dwfgpu_s* h, double mass, int dag) {
 
 MobArgs mob_args(const dwfgpu_s* h, double mass, int dag) {
   MobArgs a;
+  a.active = nullptr;
   const double kb = mob_kappa_b(h), kc = mob_kappa_c(h);
   a.b = h->b;
   a.c = h->c;
@@ -401,10 +402,12 @@ struct Ops {
 
   // Mobius hop kernel; fl: dwfk::FL_* flags
   static int mob(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, const DVec& psi,
-                 int p_out, int dag, int epi, int fl, double* partial = nullptr) {
+                 int p_out, int dag, int epi, int fl, double* partial = nullptr,
+                 const int* active = nullptr) {
     if (!partial) partial = h->partial;
     const dim3 grid(h->grid_hop), block(h->ns, h->ls);
-    const MobArgs a = mob_args(h, mass, dag);
+    MobArgs a = mob_args(h, mass, dag);
+    a.active = active;
     // the Schur passes sweep backwards: they start on the sites the pass
     // before them wrote last, which the last-level cache still holds
     Geom gm = h->g;
@@ -465,13 +468,36 @@ struct Ops {
 
   // first / second half of MatPc (dag 0) or MatPcDag (dag 1): the first writes
   // te (for dag 0 already multiplied by (b + c D5), consumed by the second)
-  static int mob_first(dwfgpu_s* h, double mass, const DVec& te, const DVec& in, int dag) {
-    return mob(h, mass, te, in, DVec(), 0, dag, dwfk::EPI_M5INV, dag ? 0 : dwfk::FL_POST_T);
+  static int mob_first(dwfgpu_s* h, double mass, const DVec& te, const DVec& in, int dag,
+                       const int* active = nullptr) {
+    return mob(h, mass, te, in, DVec(), 0, dag, dwfk::EPI_M5INV, dag ? 0 : dwfk::FL_POST_T,
+               nullptr, active);
   }
   static int mob_second(dwfgpu_s* h, double mass, const DVec& out, const DVec& te,
-                        const DVec& in, int dag, bool norm, double* partial = nullptr) {
+                        const DVec& in, int dag, bool norm, double* partial = nullptr,
+                        const int* active = nullptr) {
     const int fl = (dag ? 0 : dwfk::FL_PRE_T) | (norm && !dag ? dwfk::FL_NORM : 0);
-    return mob(h, mass, out, te, in, 1, dag, dwfk::EPI_SCHUR, fl, partial);
+    return mob(h, mass, out, te, in, 1, dag, dwfk::EPI_SCHUR, fl, partial, active);
+  }
+  // the last pass of a Mobius CG step: dagger hop of te, (b + c D5^dag) and
+  // the Schur term with t1, then r -= alpha A p and |r|^2 partials (and
+  // x += alpha p unless PX = NoX), alpha from scal[0] / scal[1]
+  template <class PX>
+  static int mob_cg_last(dwfgpu_s* h, double mass, const DVec& te, const DVec& t1,
+                         const DVec& r, const DVec& p, const DVec& x, const double* scal,
+                         double* partial, const int* active) {
+    const dim3 grid(h->grid_hop), block(h->ns, h->ls);
+    const dwfk::CgEpi<P, PX> cg{fld<P>(r), cfld<P>(p), fld<PX>(x), scal};
+    MobArgs a = mob_args(h, mass, 1);
+    a.active = active;
+    const size_t lds = std::max(link_lds_bytes(h, 1, sizeof(R)), col_lds_bytes(h, sizeof(R)));
+    Geom gm = h->g;
+    gm.rev = 1;  // a Schur pass: backwards, as in mob()
+    hipLaunchKernelGGL((dwfk::k_mobius<P, 1, dwfk::EPI_SCHUR, dwfk::FL_CG, PX>), grid, block,
+                       lds, h->st, fld<P>(te), cfld<P>(te), cfld<P>(t1), (const S*)F(h).U,
+                       partial, gm, 1, a, cg);
+    HIPCHK(hipGetLastError());
+    return DWFGPU_OK;
   }
 
   static int m5inv(dwfgpu_s* h, double mass, const DVec& out, const DVec& in, int dag) {
@@ -552,13 +578,7 @@ struct Ops {
       TRY(mob_second(h, mass, t1, te, p, 0, true));
       TRY(reduce_to(h, h->grid_hop, 1));
       TRY(mob_first(h, mass, te, t1, 1));
-      const MobArgs a = mob_args(h, mass, 1);
-      const size_t lds = std::max(link_lds_bytes(h, 1, sizeof(R)), col_lds_bytes(h, sizeof(R)));
-      Geom gm = h->g;
-      gm.rev = 1;  // a Schur pass: backwards, as in mob()
-      hipLaunchKernelGGL((dwfk::k_mobius<P, 1, dwfk::EPI_SCHUR, dwfk::FL_CG, PX>), grid, block,
-                         lds, h->st, fld<P>(te), cfld<P>(te), cfld<P>(t1), u, h->partial, gm, 1,
-                         a, cg);
+      TRY(mob_cg_last<PX>(h, mass, te, t1, r, p, x, h->dscal, h->partial, nullptr));
     }
     HIPCHK(hipGetLastError());
     return DWFGPU_OK;
@@ -722,40 +742,56 @@ struct Ops {
   // DiracOpDwf::MatInv (d_op_dwf.C:316-507) / DiracOpMobius::MatInv ORIG
   // (d_op_mobius.C:407-696) on the two device halves (first, second) of the
   // full vectors in CPS order.
+  // MatInv's source of the CG (d_op_dwf.C:366-419; d_op_mobius.C:476-486 and
+  // :546, ORIG pc) from the full vector [in_first | in_second] into src; uses
+  // work[3] and work[5] (ensure_work(h, 6) first)
+  static int matinv_src(dwfgpu_s* h, double mass, const DVec& src, const DVec& in_first,
+                        const DVec& in_second) {
+    const DVec& temp = F(h).work[3];
+    if (h->action == DWFGPU_DWF) {
+      // [odd | even]: MatPcDag(in_odd + kappa D in_even)
+      TRY(dwf(h, mass, temp, in_second, 0, 0, 1, in_first, dwf_kappa(h), false));
+    } else {
+      // [even | odd]: MatPcDag(in_odd + kappa_b D4 M5^-1 in_even)
+      const DVec& temp2 = F(h).work[5];
+      TRY(m5inv(h, mass, temp, in_first, 0));
+      TRY(mob(h, mass, temp2, temp, DVec(), 1, 0, dwfk::EPI_PLAIN, 0));
+      TRY(axpy_to(h, temp, temp2, in_second, mob_kappa_b(h), false));
+    }
+    return matpc(h, mass, src, temp, 1, false);
+  }
+  // the half of the solution the CG did not give (DWF: even from odd;
+  // Mobius: even from odd, then both halves times 2 kappa_b), d_op_dwf.C:
+  // 489-494, d_op_mobius.C:648-677; uses work[3]
+  static int matinv_sol(dwfgpu_s* h, double mass, const DVec& out_first,
+                        const DVec& out_second, const DVec& in_first, const DVec& in_second) {
+    const DVec& temp = F(h).work[3];
+    if (h->action == DWFGPU_DWF)
+      return dwf(h, mass, out_second, out_first, 1, 0, 1, in_second, dwf_kappa(h), false);
+    const double kb = mob_kappa_b(h);
+    TRY(mob(h, mass, temp, out_second, DVec(), 0, 0, dwfk::EPI_PLAIN, 0));
+    TRY(scale(h, temp, kb));
+    TRY(m5inv(h, mass, out_first, temp, 0));
+    TRY(m5inv(h, mass, temp, in_first, 0));
+    TRY(axpby(h, out_first, temp, 1.0, 1.0));
+    TRY(scale(h, out_first, 2.0 * kb));
+    return scale(h, out_second, 2.0 * kb);
+  }
+  // the CG half of a full vector: DWF [odd | even] -> first, Mobius
+  // [even | odd] -> second
+  static const DVec& cg_half(const dwfgpu_s* h, const DVec& first, const DVec& second) {
+    return h->action == DWFGPU_DWF ? first : second;
+  }
+
   static int matinv(dwfgpu_s* h, const dwfgpu_cg_arg* arg, const DVec& out_first,
                     const DVec& out_second, const DVec& in_first, const DVec& in_second,
                     double* true_res, int* iters) {
     TRY(ensure_work<P>(h, 6));
-    const double mass = arg->mass;
-    const DVec& temp = F(h).work[3];
     const DVec& src = F(h).work[4];
-    int st;
-    if (h->action == DWFGPU_DWF) {
-      const double kappa = dwf_kappa(h);
-      // [odd | even]
-      TRY(dwf(h, mass, temp, in_second, 0, 0, 1, in_first, kappa, false));
-      TRY(matpc(h, mass, src, temp, 1, false));
-      st = invcg(h, arg, out_first, src, true_res, iters);
-      if (st != DWFGPU_OK && st != DWFGPU_MAXITER) return st;
-      TRY(dwf(h, mass, out_second, out_first, 1, 0, 1, in_second, kappa, false));
-    } else {
-      // [even | odd]
-      const double kb = mob_kappa_b(h);
-      const DVec& temp2 = F(h).work[5];
-      TRY(m5inv(h, mass, temp, in_first, 0));
-      TRY(mob(h, mass, temp2, temp, DVec(), 1, 0, dwfk::EPI_PLAIN, 0));
-      TRY(axpy_to(h, temp, temp2, in_second, kb, false));
-      TRY(matpc(h, mass, src, temp, 1, false));
-      st = invcg(h, arg, out_second, src, true_res, iters);
-      if (st != DWFGPU_OK && st != DWFGPU_MAXITER) return st;
-      TRY(mob(h, mass, temp, out_second, DVec(), 0, 0, dwfk::EPI_PLAIN, 0));
-      TRY(scale(h, temp, kb));
-      TRY(m5inv(h, mass, out_first, temp, 0));
-      TRY(m5inv(h, mass, temp, in_first, 0));
-      TRY(axpby(h, out_first, temp, 1.0, 1.0));
-      TRY(scale(h, out_first, 2.0 * kb));
-      TRY(scale(h, out_second, 2.0 * kb));
-    }
+    TRY(matinv_src(h, arg->mass, src, in_first, in_second));
+    const int st = invcg(h, arg, cg_half(h, out_first, out_second), src, true_res, iters);
+    if (st != DWFGPU_OK && st != DWFGPU_MAXITER) return st;
+    TRY(matinv_sol(h, arg->mass, out_first, out_second, in_first, in_second));
     return st;
   }
 
@@ -1652,6 +1688,12 @@ size_t mr_lds(const dwfgpu_s* h, const MrShape& t, size_t creal) {
 // fewer sites per work-group than that
 constexpr int MR_MIN_NS = 8;
 int mr_partials_per_rhs(const dwfgpu_s* h) { return (h->g.vh + MR_MIN_NS - 1) / MR_MIN_NS; }
+// partials per RHS of the batched buffers: the DWF shapes' bound, or the
+// Mobius passes' grid (the handle's hop shape, which may hold fewer than
+// MR_MIN_NS sites per work-group at long Ls)
+int mr_partials_stride(const dwfgpu_s* h) {
+  return std::max(mr_partials_per_rhs(h), h->grid_hop);
+}
 
 bool mr_shape_ok(const dwfgpu_s* h, const MrShape& t, int nrhs, size_t creal) {
   return t.ns >= MR_MIN_NS && t.nr >= 1 && t.nr <= nrhs && t.ns * h->ls * t.nr <= 512 &&
@@ -1741,6 +1783,50 @@ int mr_mdagm(dwfgpu_s* h, const MrShape& t, int nrhs, const DVec& out, const DVe
   } else {
     a.out = fld<P>(out);
     TRY(mr_launch<P>(h, t, nrhs, a, 4));
+  }
+  return DWFGPU_OK;
+}
+
+// batched Mobius MatPcDagMatPc: the four hop passes of each RHS with its own
+// mass, launched per RHS (the M5^-1 and (b + c D5) epilogues keep every s-
+// column of a site in one work-group's LDS, so RHS share a launch shape but
+// not a work-group).  A paused RHS (active[j] == 0) costs four launches whose
+// work-groups leave at once.  Partials: grid_hop per RHS, RHS j at j*grid_hop.
+// cg: the CG step of every active RHS (r -= alpha A p, x += alpha p in the
+// last pass, alpha and |r|^2 on the device) instead of `out`.
+template <class P>
+int mr_mdagm_mob(dwfgpu_s* h, int nrhs, const DVec& out, const DVec& in, const DVec& te,
+                 const DVec& t1, const double* masses, const int* active, double* partial,
+                 double* scal, const DVec* r, const DVec* x, bool cg) {
+  typedef Ops<P> O;
+  const int gx = h->grid_hop;
+  for (int j = 0; j < nrhs; ++j) {
+    const DVec inj = mslice<P>(h, in, j), tej = mslice<P>(h, te, j), t1j = mslice<P>(h, t1, j);
+    TRY(O::mob_first(h, masses[j], tej, inj, 0, active + j));
+    TRY(O::mob_second(h, masses[j], t1j, tej, inj, 0, scal != nullptr, partial + (size_t)j * gx,
+                      active + j));
+  }
+  if (scal) {
+    hipLaunchKernelGGL(dwfk::k_reduce_mrhs, dim3(nrhs), dim3(1024), 0, h->st, partial, gx, scal,
+                       (int)dwfk::MR_D, active);
+    HIPCHK(hipGetLastError());
+  }
+  for (int j = 0; j < nrhs; ++j) {
+    const DVec tej = mslice<P>(h, te, j), t1j = mslice<P>(h, t1, j);
+    TRY(O::mob_first(h, masses[j], tej, t1j, 1, active + j));
+    if (cg)
+      TRY(O::template mob_cg_last<P32>(h, masses[j], tej, t1j, mslice<P>(h, *r, j),
+                                       mslice<P>(h, in, j), mslice<P32>(h, *x, j),
+                                       scal + (size_t)j * dwfk::MR_NSCAL,
+                                       partial + (size_t)j * gx, active + j));
+    else
+      TRY(O::mob_second(h, masses[j], mslice<P>(h, out, j), tej, t1j, 1, false, nullptr,
+                        active + j));
+  }
+  if (cg) {
+    hipLaunchKernelGGL(dwfk::k_reduce_mrhs, dim3(nrhs), dim3(1024), 0, h->st, partial, gx, scal,
+                       (int)dwfk::MR_RRNEW, active);
+    HIPCHK(hipGetLastError());
   }
   return DWFGPU_OK;
 }
@@ -1842,7 +1928,7 @@ int invcg_mrhs(dwfgpu_s* h, const dwfgpu_cg_arg* arg, int nrhs, const double* ma
   const DVec& ax = h->f[DWFGPU_F64].work[1];
   MrState<PS> m;
   m.nrhs = nrhs;
-  m.gx = mr_partials_per_rhs(h);  // partials sized for the smallest allowed ns
+  m.gx = mr_partials_stride(h);
   TRY(alloc_mvec<PS>(h, nrhs, &m.te));
   TRY(alloc_mvec<PS>(h, nrhs, &m.t1));
   TRY(alloc_mvec<PS>(h, nrhs, &m.rs));
@@ -1867,8 +1953,8 @@ int invcg_mrhs(dwfgpu_s* h, const dwfgpu_cg_arg* arg, int nrhs, const double* ma
     std::vector<int> ones(nrhs, 1);
     HIPCHK(hipMemcpyAsync(m.ibuf, ones.data(), nrhs * sizeof(int), hipMemcpyHostToDevice, h->st));
   }
-  MrShape t{};
-  TRY(mr_tune<PS>(h, nrhs, m, m.active(), &t));
+  MrShape t{h->ns, 1};  // Mobius: the handle's own hop shape, one RHS per work-group
+  if (h->action == DWFGPU_DWF) TRY(mr_tune<PS>(h, nrhs, m, m.active(), &t));
   h->mr_last_ns = t.ns;
   h->mr_last_nr = t.nr;
 
@@ -1908,8 +1994,12 @@ int invcg_mrhs(dwfgpu_s* h, const dwfgpu_cg_arg* arg, int nrhs, const double* ma
 
   const size_t vs = h->n / 2 / PS::kCPC;
   auto step = [&]() -> int {
-    TRY(mr_mdagm<PS>(h, t, nrhs, DVec(), m.ps, m.te, m.t1, m.mass, m.active(), m.partial, m.scal,
-                     &m.rs, &m.xs, true));
+    if (h->action == DWFGPU_DWF)
+      TRY(mr_mdagm<PS>(h, t, nrhs, DVec(), m.ps, m.te, m.t1, m.mass, m.active(), m.partial,
+                       m.scal, &m.rs, &m.xs, true));
+    else
+      TRY(mr_mdagm_mob<PS>(h, nrhs, DVec(), m.ps, m.te, m.t1, masses, m.active(), m.partial,
+                           m.scal, &m.rs, &m.xs, true));
     hipLaunchKernelGGL(dwfk::k_cg_scalars_mrhs, dim3(1), dim3(256), 0, h->st, m.scal, m.active(),
                        m.flag(), m.iters(), nrhs);
     HIPCHK(hipGetLastError());
@@ -3453,17 +3543,13 @@ int dwfgpu_matpcdagmatpc_mrhs_dev(dwfgpu_t h, int precision, const double* masse
                                   dwfgpu_mvec_t out, dwfgpu_mvec_t in) {
   if (!h || !masses || !out || !in || out->h != h || in->h != h || out->nrhs != in->nrhs)
     return DWFGPU_BAD_ARG;
-  if (h->action != DWFGPU_DWF) {
-    set_err("dwfgpu_matpcdagmatpc_mrhs_dev: the batched operator is DWF only");
-    return DWFGPU_BAD_ARG;
-  }
   const int nrhs = in->nrhs;
   auto run = [&](auto tag) -> int {
     typedef decltype(tag) P;
     TRY(ensure_fields<P>(h));
     MrState<P> m;
     m.nrhs = nrhs;
-    m.gx = mr_partials_per_rhs(h);
+    m.gx = mr_partials_stride(h);
     TRY(alloc_mvec<P>(h, nrhs, &m.te));
     TRY(alloc_mvec<P>(h, nrhs, &m.t1));
     TRY(alloc_mvec<P>(h, nrhs, &m.rs));
@@ -3478,12 +3564,17 @@ int dwfgpu_matpcdagmatpc_mrhs_dev(dwfgpu_t h, int precision, const double* masse
       hipLaunchKernelGGL((dwfk::k_convert<P, P64>), blas_grid(), blas_block(), 0, h->st,
                          fld<P>(mslice<P>(h, m.ps, j)), cfld<P64>(mslice<P64>(h, in->v, j)), h->g);
     HIPCHK(hipGetLastError());
-    MrShape t{};
-    TRY(mr_tune<P>(h, nrhs, m, m.active(), &t));
+    MrShape t{h->ns, 1};
+    if (h->action == DWFGPU_DWF) {
+      TRY(mr_tune<P>(h, nrhs, m, m.active(), &t));
+      TRY(mr_mdagm<P>(h, t, nrhs, m.rs, m.ps, m.te, m.t1, m.mass, m.active(), m.partial,
+                      nullptr, nullptr, nullptr, false));
+    } else {
+      TRY(mr_mdagm_mob<P>(h, nrhs, m.rs, m.ps, m.te, m.t1, masses, m.active(), m.partial,
+                          nullptr, nullptr, nullptr, false));
+    }
     h->mr_last_ns = t.ns;
     h->mr_last_nr = t.nr;
-    TRY(mr_mdagm<P>(h, t, nrhs, m.rs, m.ps, m.te, m.t1, m.mass, m.active(), m.partial, nullptr,
-                    nullptr, nullptr, false));
     for (int j = 0; j < nrhs; ++j)
       hipLaunchKernelGGL((dwfk::k_convert<P64, P>), blas_grid(), blas_block(), 0, h->st,
                          fld<P64>(mslice<P64>(h, out->v, j)), cfld<P>(mslice<P>(h, m.rs, j)), h->g);
@@ -3506,10 +3597,7 @@ int dwfgpu_invcg_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, doub
   if (!h || !arg || !masses || !sol || !src || !true_res || !iters || !reliable_updates ||
       !status || sol->h != h || src->h != h || sol->nrhs != src->nrhs)
     return DWFGPU_BAD_ARG;
-  if (h->action != DWFGPU_DWF) {
-    set_err("dwfgpu_invcg_mrhs_dev: the batched solver is DWF only");
-    return DWFGPU_BAD_ARG;
-  }
+
   if (arg->max_num_iter < 1 || !(delta > 0.0 && delta < 1.0)) {
     set_err("dwfgpu_invcg_mrhs_dev: need max_num_iter >= 1 and 0 < delta < 1");
     return DWFGPU_BAD_ARG;
@@ -3529,6 +3617,68 @@ int dwfgpu_invcg_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, doub
       set_err("dwfgpu_invcg_mrhs_dev: sloppy precision must be F32, BF16 or I16");
       return DWFGPU_BAD_ARG;
   }
+}
+
+int dwfgpu_matinv_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, double delta,
+                           int flags, const double* masses, dwfgpu_mvec_t out_first,
+                           dwfgpu_mvec_t out_second, dwfgpu_mvec_t in_first,
+                           dwfgpu_mvec_t in_second, double* true_res, int* iters,
+                           int* reliable_updates, int* status) {
+  if (!h || !arg || !masses || !out_first || !out_second || !in_first || !in_second ||
+      !true_res || !iters || !reliable_updates || !status)
+    return DWFGPU_BAD_ARG;
+  const int nrhs = in_first->nrhs;
+  for (dwfgpu_mvec_t v : {out_first, out_second, in_first, in_second})
+    if (v->h != h || v->nrhs != nrhs) {
+      set_err("dwfgpu_matinv_mrhs_dev: the four vectors need this handle and one RHS count");
+      return DWFGPU_BAD_ARG;
+    }
+  if (arg->max_num_iter < 1 || !(delta > 0.0 && delta < 1.0)) {
+    set_err("dwfgpu_matinv_mrhs_dev: need max_num_iter >= 1 and 0 < delta < 1");
+    return DWFGPU_BAD_ARG;
+  }
+  if (sloppy != DWFGPU_F32 && sloppy != DWFGPU_BF16 && sloppy != DWFGPU_I16) {
+    set_err("dwfgpu_matinv_mrhs_dev: sloppy precision must be F32, BF16 or I16");
+    return DWFGPU_BAD_ARG;
+  }
+  typedef Ops<P64> O;
+  TRY(ensure_work<P64>(h, 6));
+  struct Src {  // the CG sources of all RHS, freed on every return
+    DVec v;
+    ~Src() { free_vec(&v); }
+  } src;
+  TRY(alloc_mvec<P64>(h, nrhs, &src.v));
+  // per RHS: the CG source from [in_first | in_second]
+  for (int j = 0; j < nrhs; ++j)
+    TRY(O::matinv_src(h, masses[j], mslice<P64>(h, src.v, j), mslice<P64>(h, in_first->v, j),
+                      mslice<P64>(h, in_second->v, j)));
+  const DVec& sol = O::cg_half(h, out_first->v, out_second->v);
+  int st;
+  switch (sloppy) {
+    case DWFGPU_F32:
+      st = invcg_mrhs<P32>(h, arg, nrhs, masses, delta, flags, sol, src.v, true_res, iters,
+                           reliable_updates, status);
+      break;
+    case DWFGPU_BF16:
+      st = invcg_mrhs<PBF16>(h, arg, nrhs, masses, delta, flags, sol, src.v, true_res, iters,
+                             reliable_updates, status);
+      break;
+    default:
+      st = invcg_mrhs<PI16>(h, arg, nrhs, masses, delta, flags, sol, src.v, true_res, iters,
+                            reliable_updates, status);
+      break;
+  }
+  if (st) {
+    (void)hipStreamSynchronize(h->st);
+    return st;
+  }
+  // per RHS: the other half of the solution
+  for (int j = 0; j < nrhs; ++j)
+    TRY(O::matinv_sol(h, masses[j], mslice<P64>(h, out_first->v, j),
+                      mslice<P64>(h, out_second->v, j), mslice<P64>(h, in_first->v, j),
+                      mslice<P64>(h, in_second->v, j)));
+  HIPCHK(hipStreamSynchronize(h->st));
+  return DWFGPU_OK;
 }
 
 int dwfgpu_mrhs_shape(dwfgpu_t h, int* ns, int* nr) {

@@ -1042,6 +1042,9 @@ struct MobArgs {
   double kappa_b;
   float fb, fc, fmc, fK, fmK, fkappa_b;
   M5Side m5;
+  // batched solves: the active flag of this launch's right-hand side
+  // (nullptr: always run); a paused one's work-groups leave at once
+  const int* active;
 };
 
 // the coefficient in the compute type R
@@ -1193,6 +1196,7 @@ __global__ void __launch_bounds__(512)
 k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ U,
          double* __restrict__ partial, Geom g, int p_out, MobArgs a, CgEpi<P, PX> cg) {
   typedef typename P::R R;
+  if (a.active && *a.active == 0) return;  // uniform: before any barrier
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __shared__ double sh[16];
   const int ns = blockDim.x, il = threadIdx.x, s = threadIdx.y;

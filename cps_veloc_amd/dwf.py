@@ -45,6 +45,7 @@ EXPORTS = (
     "dwfgpu_invcg_mixed", "dwfgpu_invcg_mixed_dev",
     "dwfgpu_mvec_create", "dwfgpu_mvec_destroy", "dwfgpu_mvec_upload", "dwfgpu_mvec_download",
     "dwfgpu_matpcdagmatpc_mrhs_dev", "dwfgpu_invcg_mrhs_dev", "dwfgpu_mrhs_shape",
+    "dwfgpu_matinv_mrhs_dev",
     "dwfgpu_ptc_num_weights", "dwfgpu_ptc_init", "dwfgpu_ptc_set_weights",
     "dwfgpu_ptc_get_weights", "dwfgpu_ptc_apply", "dwfgpu_ptc_loss_grad", "dwfgpu_ptc_train",
     "dwfgpu_invcg_ptc_dev", "dwfgpu_minvcg", "dwfgpu_minvcg_dev",
@@ -157,6 +158,8 @@ def lib():
         L.dwfgpu_invcg_mrhs_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, C.c_int,
                                             _dp, H, H, _dp, _ip, _ip, _ip]
         L.dwfgpu_mrhs_shape.argtypes = [H, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.dwfgpu_matinv_mrhs_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, C.c_int,
+                                             _dp, H, H, H, H, _dp, _ip, _ip, _ip]
         _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
         L.dwfgpu_ptc_num_weights.argtypes = [H]
         L.dwfgpu_ptc_num_weights.restype = C.c_size_t
@@ -501,6 +504,24 @@ class _DiracOpGpu:
                                              src.h, C.byref(tr), C.byref(it), C.byref(nr),
                                              C.byref(ns)), "InvCgPtc_dev", allow=(OK, MAXITER))
         return it.value, tr.value, st, nr.value, ns.value
+
+    def MatInvMrhs_dev(self, out_first: MultiVector, out_second: MultiVector,
+                       in_first: MultiVector, in_second: MultiVector, cg_arg: CgArg, masses,
+                       sloppy="f32", delta=0.1, eager=False):
+        """DiracOpMobius::MatInv(vector) (and DWF): full vectors [first | second]
+        of every RHS, one batched CG; returns per-RHS arrays
+        (iters, true_res, status, reliable_updates)."""
+        n = in_first.nrhs
+        tr = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        nr = np.zeros(n, dtype=np.int32)
+        stv = np.zeros(n, dtype=np.int32)
+        a = cg_arg.c()
+        _chk(lib().dwfgpu_matinv_mrhs_dev(self.h, C.byref(a), _PREC[sloppy], delta,
+                                          1 if eager else 0, _arr(masses), out_first.h,
+                                          out_second.h, in_first.h, in_second.h, tr, it, nr,
+                                          stv), "MatInvMrhs_dev")
+        return it, tr, stv, nr
 
     def mrhs_shape(self):
         ns, nr = C.c_int(), C.c_int()

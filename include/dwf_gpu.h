@@ -172,11 +172,13 @@ int dwfgpu_mvec_create(dwfgpu_t h, int nrhs, dwfgpu_mvec_t *out);
 int dwfgpu_mvec_destroy(dwfgpu_mvec_t v);
 int dwfgpu_mvec_upload(dwfgpu_mvec_t v, int j, const double *host_cps_order);
 int dwfgpu_mvec_download(dwfgpu_mvec_t v, int j, double *host_cps_order);
-/* MatPcDagMatPc of every RHS j with mass masses[j], applied in `precision`
- * by the batched operator (one launch per hopping pass for all RHS, links
- * staged once per work-group for all of them); DWF handles only.  The
- * per-RHS loop over DiracOpDwf::MatPcDagMatPc (d_op_dwf.C:174-200) that CPS
- * callers such as QPropW (QPropW.C:1236) run, in one pass.                   */
+/* MatPcDagMatPc of every RHS j with mass masses[j], applied in `precision`.
+ * DWF: the batched operator (one launch per hopping pass for all RHS, links
+ * staged once per work-group for all of them).  Mobius: the four passes of
+ * each RHS (their M5^-1 / (b + c D5) epilogues keep a site's s-column in one
+ * work-group) issued back to back.  The per-RHS loop over
+ * DiracOpDwf/DiracOpMobius::MatPcDagMatPc (d_op_dwf.C:174-200,
+ * d_op_mobius.C:200-218) that CPS callers such as QPropW (QPropW.C:1236) run. */
 int dwfgpu_matpcdagmatpc_mrhs_dev(dwfgpu_t h, int precision, const double *masses,
                                   dwfgpu_mvec_t out, dwfgpu_mvec_t in);
 /* InvCg's solve (inv_cg.C:84-448 stop rule per RHS) for every RHS j with its
@@ -193,6 +195,21 @@ int dwfgpu_invcg_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, doub
                           double *true_res, int *iters, int *reliable_updates, int *status);
 /* the launch shape (sites, RHS per work-group) the last batched call used */
 int dwfgpu_mrhs_shape(dwfgpu_t h, int *ns, int *nr);
+/* DiracOpMobius::MatInv(std::vector<Vector*> out, std::vector<Vector*> in,
+ * true_res, iters) (d_op_mobius.C:407-696, ORIG pc; the per-RHS InvCg of
+ * :552-554 becomes one batched solve) and the same for DWF handles (the
+ * DiracOpDwf::MatInv of d_op_dwf.C:316-507 per RHS): for every RHS j, full
+ * vectors [in_first | in_second] -> [out_first | out_second] as
+ * dwfgpu_matinv_dev (DWF [5D-odd | 5D-even], Mobius [4D-even | 4D-odd]),
+ * with mass masses[j] (CPS's vector form uses one mass for all).  The CG
+ * sources are prepared and the solutions completed on the device; the solve
+ * is dwfgpu_invcg_mrhs_dev (sloppy, delta, flags and the per-RHS outputs as
+ * there; the CG half of out holds the initial guess on entry).             */
+int dwfgpu_matinv_mrhs_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, double delta,
+                           int flags, const double *masses, dwfgpu_mvec_t out_first,
+                           dwfgpu_mvec_t out_second, dwfgpu_mvec_t in_first,
+                           dwfgpu_mvec_t in_second, double *true_res, int *iters,
+                           int *reliable_updates, int *status);
 
 /* ---- multi-shift CG ------------------------------------------------------ */
 /* DiracOp::MInvCG(psi, chi, chi_norm, shift, Nshift, isz, RsdCG, type, alpha)

@@ -1,4 +1,4 @@
-"""GPU: the batched many-RHS DWF operator and CG.
+"""GPU: the batched many-RHS operator, CG and MatInv (DWF and Mobius).
 
 Bars
   * batched FP64 MatPcDagMatPc of every RHS == the single-RHS FP64 operator
@@ -10,7 +10,10 @@ Bars
     solve to 1e-8, masses give different iteration counts (RHS retire on their
     own), its count is within 2 of the single-RHS solver's and within 3 of the
     oracle's, and the graph-replayed solve equals the eagerly launched one bit
-    for bit, iteration counts included.
+    for bit, iteration counts included;
+  * batched MatInv (DiracOpMobius::MatInv's vector form, and DWF): every RHS's
+    full solution within 1e-8 of the single-RHS MatInv (FP64 CG) and the
+    sources left unchanged.
 """
 import sys
 
@@ -39,18 +42,22 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def setup(dims, ls, nrhs, seed=3):
+def setup(dims, ls, nrhs, seed=3, action="dwf"):
     U = lat.random_gauge(dims, 0.3, 101)
-    g = DiracOpDwfGpu(U, dims, ls, m5=M5)
-    o = Oracle(dims, ls, "dwf", m5=M5, gauge_canonical=U)
+    if action == "dwf":
+        g = DiracOpDwfGpu(U, dims, ls, m5=M5)
+    else:
+        g = DiracOpMobiusGpu(U, dims, ls, m5=M5, b=1.5, c=0.5)
+    o = Oracle(dims, ls, action, m5=M5, b=1.5, c=0.5, gauge_canonical=U)
     xs = [src_vec(dims, ls, seed + j) for j in range(nrhs)]
     return g, o, xs
 
 
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
 @pytest.mark.parametrize("dims,ls,nrhs", [((4, 4, 4, 8), 8, 5), ((4, 2, 6, 4), 6, 3)])
-def test_batched_operator(dims, ls, nrhs):
-    say(f"[gpu] batched MatPcDagMatPc {dims} ls={ls} nrhs={nrhs}")
-    g, o, xs = setup(dims, ls, nrhs)
+def test_batched_operator(dims, ls, nrhs, action):
+    say(f"[gpu] batched MatPcDagMatPc {action} {dims} ls={ls} nrhs={nrhs}")
+    g, o, xs = setup(dims, ls, nrhs, action=action)
     masses = np.array([0.01, 0.04, 0.3, 0.005, 0.1][:nrhs])
     vin, vout = g.multi_vector(nrhs, xs), g.multi_vector(nrhs)
     g.MatPcDagMatPc_mrhs_dev(vout, vin, masses, "f64")
@@ -65,13 +72,14 @@ def test_batched_operator(dims, ls, nrhs):
             assert rel(vout.download(j), want) < tol, (prec, j)
 
 
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
 @pytest.mark.parametrize("sloppy", ["f32", "i16"])
-def test_batched_cg(sloppy):
+def test_batched_cg(sloppy, action):
     dims, ls = (4, 4, 4, 8), 8
     masses = np.array([0.01, 0.04, 0.5, 0.04])
     nrhs = len(masses)
-    say(f"[gpu] batched CG sloppy={sloppy} nrhs={nrhs} masses={masses.tolist()}")
-    g, o, xs = setup(dims, ls, nrhs, seed=11)
+    say(f"[gpu] batched CG {action} sloppy={sloppy} nrhs={nrhs} masses={masses.tolist()}")
+    g, o, xs = setup(dims, ls, nrhs, seed=11, action=action)
     arg = CgArg(0.0, 5000, 1e-10)
     res = {}
     for eager in (False, True):
@@ -116,9 +124,42 @@ def test_batched_cg_maxiter_and_args():
     with pytest.raises(DwfGpuError) as e:
         g.InvCgMrhs_dev(sol, src, CgArg(0.0, 10, 1e-8), np.array([0.1, 0.1]), "f32", delta=1.5)
     assert e.value.code == BAD_ARG
-    U = lat.random_gauge(dims, 0.3, 1)
-    mob = DiracOpMobiusGpu(U, dims, ls, m5=M5)
-    a, b = mob.multi_vector(1), mob.multi_vector(1)
+    # MatInv: the four vectors must share one RHS count
+    a, b = g.multi_vector(2), g.multi_vector(1)
     with pytest.raises(DwfGpuError) as e:
-        mob.MatPcDagMatPc_mrhs_dev(a, b, np.array([0.1]))
+        g.MatInvMrhs_dev(a, a, a, b, CgArg(0.0, 10, 1e-8), np.array([0.1, 0.1]), "f32")
     assert e.value.code == BAD_ARG
+
+
+@pytest.mark.parametrize("action", ["dwf", "mobius"])
+def test_batched_matinv(action):
+    """Full-vector solves of several RHS in one batched CG against the
+    single-RHS MatInv (FP64 InvCg) of each."""
+    dims, ls = (4, 4, 4, 8), 8
+    masses = np.array([0.04, 0.04, 0.1])
+    nrhs = len(masses)
+    say(f"[gpu] batched MatInv {action} nrhs={nrhs}")
+    g, o, _ = setup(dims, ls, nrhs, action=action)
+    firsts = [src_vec(dims, ls, 40 + j) for j in range(nrhs)]
+    seconds = [src_vec(dims, ls, 50 + j) for j in range(nrhs)]
+    inf, ins = g.multi_vector(nrhs, firsts), g.multi_vector(nrhs, seconds)
+    outf, outs = g.multi_vector(nrhs), g.multi_vector(nrhs)
+    it, tr, st, nrel = g.MatInvMrhs_dev(outf, outs, inf, ins, CgArg(0.0, 5000, 1e-10), masses,
+                                        "f32", 0.1)
+    say(f"  batched iters {it.tolist()} true_res {np.array2string(tr, precision=2)}")
+    assert np.all(st == 0) and np.all(tr <= 1e-10)
+    v = [g.vector() for _ in range(4)]
+    for j in range(nrhs):
+        assert np.array_equal(inf.download(j), firsts[j])   # sources preserved
+        assert np.array_equal(ins.download(j), seconds[j])
+        v[0].upload(np.zeros(g.n))
+        v[1].upload(np.zeros(g.n))
+        v[2].upload(firsts[j])
+        v[3].upload(seconds[j])
+        i1, tr1, st1 = g.MatInv_dev(v[0], v[1], v[2], v[3], CgArg(masses[j], 5000, 1e-10))
+        assert st1 == 0
+        want = np.concatenate([v[0].download(), v[1].download()])
+        got = np.concatenate([outf.download(j), outs.download(j)])
+        err = rel(got, want)
+        say(f"  RHS {j}: batched {it[j]} / single {i1} iterations, |x - x_single|/|x| {err:.2e}")
+        assert err < 1e-8 and abs(int(it[j]) - i1) <= 3
