@@ -513,9 +513,10 @@ template <int SG>
 __device__ __forceinline__ i16x2 pk_addr(i16x2 a, i16x2 b) {  // a + SG b
   return SG > 0 ? a + b : a - b;
 }
+// the six projected components, still packed (re | im << 16)
 template <int MU, int SG>
-__device__ __forceinline__ void project_pk(const uint32_t* w, cx<float>* h) {
-  i16x2 p[12], q[6];
+__device__ __forceinline__ void project_words(const uint32_t* w, i16x2* q) {
+  i16x2 p[12];
 #pragma unroll
   for (int c = 0; c < 12; ++c) p[c] = __builtin_bit_cast(i16x2, w[c]);
 #pragma unroll
@@ -534,6 +535,11 @@ __device__ __forceinline__ void project_pk(const uint32_t* w, cx<float>* h) {
       q[3 + c] = pk_addr<SG>(p[3 + c], p[9 + c]);
     }
   }
+}
+template <int MU, int SG>
+__device__ __forceinline__ void project_pk(const uint32_t* w, cx<float>* h) {
+  i16x2 q[6];
+  project_words<MU, SG>(w, q);
 #pragma unroll
   for (int c = 0; c < 6; ++c) h[c] = cx<float>{(float)q[c].x, (float)q[c].y};
 }
@@ -566,6 +572,15 @@ struct PlainLoader {
 #pragma unroll
     for (int k = 0; k < nch<P>(); ++k)
       chunk_unpack<P>(d[(size_t)k * vh + n], 1.0f, p + k * P::kCPC);
+    return ld_scale<P>(sc, n);
+  }
+  // int16 only: the projected two-spinor as packed int16 words (scale returned)
+  template <int MU, int SG>
+  __device__ __forceinline__ float proj_words(int n, i16x2* q) const {
+    static_assert(std::is_same<P, PI16>::value, "packed words are the int16 format");
+    uint32_t w[12];
+    ld_words_i16(d, vh, n, w);
+    project_words<MU, SG>(w, q);
     return ld_scale<P>(sc, n);
   }
   template <int MU, int SG>
@@ -843,6 +858,147 @@ __device__ __forceinline__ void hop4_tx(cx<R>* acc, LD ld, const cx<R>* lu, int 
   hop_dir_tx<3, 1, false>(acc, ld, nb.bwd[3], lu + 7 * ds, mine, prv, nxt, b, cm, cp);
 }
 
+// ---------------------------------------------------------------------------
+// int16 hop on integer dot products
+// ---------------------------------------------------------------------------
+// The int16 kernels whose input needs no (b + c D5) on the fly multiply the
+// projected two-spinor by the link in exact integer arithmetic: each complex
+// product is two v_dot2_i32_i16 (re and im of U h from packed int16 words)
+// instead of four FP32 multiply-adds, and the six products of a row sum in
+// int32.  Links are staged as packed words p = (Ur, Ui) and q = (Ur, -Ui) of U
+// scaled by 16383.5 (value = word * kLinkPQScale), so every partial sum of a
+// row times a projected colour vector h (|h| <= sqrt(6) * 32766, int16 spinors
+// stored in [-16383, 16383]) is below 16384 * 80260 < 1.32e9 < 2^31: no
+// intermediate can overflow, the sums are exact, and one conversion per
+// result component takes them to FP32.
+struct LinkPQ {
+  uint32_t p, q;
+};
+constexpr float kLinkPQScale = 2.0f / 32767.0f;
+
+template <class P>
+__device__ __forceinline__ LinkPQ link_pq(cx<float> u) {
+  // cvt_pknorm_i16 rounds clamp(x, -1, 1) * 32767; x = u / 2
+  const i16x2 pk = __builtin_amdgcn_cvt_pknorm_i16(0.5f * u.re, 0.5f * u.im);
+  const i16x2 qk = pk * i16x2{1, -1};
+  return LinkPQ{__builtin_bit_cast(uint32_t, pk), __builtin_bit_cast(uint32_t, qk)};
+}
+
+// stage_links for the integer hop: the same links and neighbour indices, the
+// elements as LinkPQ (8 bytes each, the footprint of the FP32 staging)
+__device__ __forceinline__ void stage_links_pq(LinkPQ* __restrict__ lu, int* __restrict__ lnb,
+                                               const uint32_t* __restrict__ U, const Geom& g,
+                                               int blk, int ns, int nsets, int p_first) {
+  constexpr int NE = PI16::kLinkElems;
+  const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
+  const int nthr = blockDim.x * blockDim.y * blockDim.z;
+  for (int w = tid; w < nsets * 8 * ns; w += nthr) {
+    const int il = w % ns, d = (w / ns) % 8, set = w / (8 * ns);
+    const int i = blk * ns + il;
+    if (i >= g.vh) continue;
+    const int p = (p_first + set) & 1;
+    const int mu = d & 3;
+    const int e = g.nbt[(size_t)(p * 8 + d) * g.vh + i];
+    const int nbr = e & 0x7fffffff;
+    const bool neg = e < 0;
+    const int site = d >= 4 ? nbr : i;
+    lnb[w] = nbr;
+    const int par = d >= 4 ? 1 - p : p;
+    const uint32_t* src = U + (size_t)((mu * 2 + par) * NE) * g.vh + site;
+    cx<float> u[9];
+#pragma unroll
+    for (int k = 0; k < NE; ++k)
+      u[k] = Pack<PI16>::unpack(src[(size_t)k * g.vh], link_scale<PI16>());
+    // row 2 = conj(row0 x row1) times the boundary sign
+    const cx<float> r0 = cmul(u[1], u[5]) - cmul(u[2], u[4]);
+    const cx<float> r1 = cmul(u[2], u[3]) - cmul(u[0], u[5]);
+    const cx<float> r2 = cmul(u[0], u[4]) - cmul(u[1], u[3]);
+    const float sg = neg ? -1.0f : 1.0f;
+    u[6] = cx<float>{sg * r0.re, -sg * r0.im};
+    u[7] = cx<float>{sg * r1.re, -sg * r1.im};
+    u[8] = cx<float>{sg * r2.re, -sg * r2.im};
+    LinkPQ* dst = lu + (size_t)w * 9;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dst[k] = link_pq<PI16>(u[k]);
+  }
+}
+
+__device__ __forceinline__ i16x2 swap16(i16x2 a) { return __builtin_shufflevector(a, a, 1, 0); }
+// c + a.lo * b.lo + a.hi * b.hi in int32.  The sums here never leave the
+// int32 range (bound above), so the clamped form gives the same bits; asking
+// for it keeps the three-operand encoding, which takes an inline zero as the
+// first accumulator and the half swap of b as an operand selector, where the
+// accumulate-in-place form needs a move and a shift per chain.
+__device__ __forceinline__ int dot2(i16x2 a, i16x2 b, int c) {
+  return __builtin_amdgcn_sdot2(a, b, c, true);
+}
+
+// one direction of the hop with integer products; lf / lb as in hop_dir
+template <int MU, int DAG>
+__device__ __forceinline__ void hop_dir_pq(cx<float>* acc, const PlainLoader<PI16>& ld,
+                                           const LinkPQ* lf, const LinkPQ* lb, const Nbr& nb) {
+  constexpr int SF = DAG ? 1 : -1;
+  i16x2 h[6];
+  cx<float> r[6];
+  {
+    // (U h)_row: re = sum_j q_rj . h_j, im = sum_j p_rj . swap(h_j)
+    const float sc = ld.template proj_words<MU, SF>(nb.fwd[MU], h);
+    LinkPQ m[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m[e] = lf[e];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int row = 0; row < 3; ++row) {
+        int re = 0, im = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const i16x2 hv = h[3 * k + j];
+          re = dot2(__builtin_bit_cast(i16x2, m[row * 3 + j].q), hv, re);
+          im = dot2(__builtin_bit_cast(i16x2, m[row * 3 + j].p), swap16(hv), im);
+        }
+        r[3 * k + row] = cx<float>{(float)re, (float)im};
+      }
+    reconstruct<MU, SF>(acc, r, sc * kLinkPQScale);
+  }
+  {
+    // (U^dag h)_row: re = sum_j p_j,row . h_j, im = sum_j q_j,row . swap(h_j)
+    const float sc = ld.template proj_words<MU, -SF>(nb.bwd[MU], h);
+    LinkPQ m[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m[e] = lb[e];
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+#pragma unroll
+      for (int row = 0; row < 3; ++row) {
+        int re = 0, im = 0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          const i16x2 hv = h[3 * k + j];
+          re = dot2(__builtin_bit_cast(i16x2, m[j * 3 + row].p), hv, re);
+          im = dot2(__builtin_bit_cast(i16x2, m[j * 3 + row].q), swap16(hv), im);
+        }
+        r[3 * k + row] = cx<float>{(float)re, (float)im};
+      }
+    reconstruct<MU, -SF>(acc, r, sc * kLinkPQScale);
+  }
+}
+
+template <int DAG>
+__device__ __forceinline__ void hop4_pq(cx<float>* acc, const PlainLoader<PI16>& ld,
+                                        const LinkPQ* lu, int ds, const int* nbi, int ns) {
+  Nbr nb;
+#pragma unroll
+  for (int mu = 0; mu < 4; ++mu) {
+    nb.fwd[mu] = nbi[mu * ns];
+    nb.bwd[mu] = nbi[(4 + mu) * ns];
+  }
+  hop_dir_pq<0, DAG>(acc, ld, lu + 0 * ds, lu + 4 * ds, nb);
+  hop_dir_pq<1, DAG>(acc, ld, lu + 1 * ds, lu + 5 * ds, nb);
+  hop_dir_pq<2, DAG>(acc, ld, lu + 2 * ds, lu + 6 * ds, nb);
+  hop_dir_pq<3, DAG>(acc, ld, lu + 3 * ds, lu + 7 * ds, nb);
+}
+
 // acc = D_hop^(dag) psi at output site i of parity p_out (wilson_dslash.C:173-517);
 // lu points at this site's 8 staged links, consecutive directions `ds` apart,
 // nbi at its 8 staged neighbour indices, `ns` apart.
@@ -966,7 +1122,11 @@ k_dwf_dslash(Fld<P> out, CFld<P> in, const typename P::S* __restrict__ U, CFld<P
   const int blk = xcd_block(g.nxcd, g.rev);
   const int i = blk * ns + il;
   // slice s holds output parity (s + 1 + cb_in) % 2: set s % 2 of the stage
-  stage_links<P>(lu, lnb, U, g, blk, ns, 2, (1 + cb_in) & 1);
+  constexpr bool kPQ = std::is_same<P, PI16>::value;  // integer hop
+  if constexpr (kPQ)
+    stage_links_pq(reinterpret_cast<LinkPQ*>(lu), lnb, U, g, blk, ns, 2, (1 + cb_in) & 1);
+  else
+    stage_links<P>(lu, lnb, U, g, blk, ns, 2, (1 + cb_in) & 1);
   __syncthreads();
   double nrm = 0.0;
   if (i < g.vh) {
@@ -974,8 +1134,13 @@ k_dwf_dslash(Fld<P> out, CFld<P> in, const typename P::S* __restrict__ U, CFld<P
     cx<R> acc[12];
 #pragma unroll
     for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
-    hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
-              lu + ((s & 1) * 8 * ns + il) * 9, ns * 9, lnb + (s & 1) * 8 * ns + il, ns);
+    const PlainLoader<P> ld{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh};
+    if constexpr (kPQ)
+      hop4_pq<DAG>(acc, ld, reinterpret_cast<const LinkPQ*>(lu) + ((s & 1) * 8 * ns + il) * 9,
+                   ns * 9, lnb + (s & 1) * 8 * ns + il, ns);
+    else
+      hop4<DAG>(acc, ld, lu + ((s & 1) * 8 * ns + il) * 9, ns * 9, lnb + (s & 1) * 8 * ns + il,
+                ns);
     // fifth dimension: the same site index i in slices s-1 / s+1
     const int sm = s > 0 ? s - 1 : g.ls - 1, spl = s < g.ls - 1 ? s + 1 : 0;
     const R cm = (R)(s > 0 ? two_a5 : nm_two_a5);
@@ -1218,7 +1383,13 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
   typename P::C* od = out.d + s * slice;
   float* osc = out.sc + (size_t)s * g.vh;
 
-  stage_links<P>(lu, lnb, U, g, blk, ns, 1, p_out);
+  // int16 passes whose input is used as it is (no (b + c D5) on the fly):
+  // integer hop on packed links
+  constexpr bool kPQ = std::is_same<P, PI16>::value && !(DAG == 0 && !(FL & FL_PRE_T));
+  if constexpr (kPQ)
+    stage_links_pq(reinterpret_cast<LinkPQ*>(lu), lnb, U, g, blk, ns, 1, p_out);
+  else
+    stage_links<P>(lu, lnb, U, g, blk, ns, 1, p_out);
   __syncthreads();
   cx<R> acc[12];
 #pragma unroll
@@ -1250,8 +1421,11 @@ k_mobius(Fld<P> out, CFld<P> in, CFld<P> psi, const typename P::S* __restrict__ 
       ld.cp = s < ls - 1 ? pick<R>(a.c, a.fc) : pick<R>(a.mc, a.fmc);
       hop4<0>(acc, ld, lu + il * 9, ns * 9, lnb + il, ns);
     } else {
-      hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
-                lu + il * 9, ns * 9, lnb + il, ns);
+      const PlainLoader<P> ld{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh};
+      if constexpr (kPQ)
+        hop4_pq<DAG>(acc, ld, reinterpret_cast<const LinkPQ*>(lu) + il * 9, ns * 9, lnb + il, ns);
+      else
+        hop4<DAG>(acc, ld, lu + il * 9, ns * 9, lnb + il, ns);
     }
   }
 
@@ -2027,7 +2201,11 @@ __global__ void __launch_bounds__(512) k_dwf_mrhs(MrArgs<P, PX> a) {
   double* red = reinterpret_cast<double*>(smem_raw + 2 * 8 * ns * 4 + 2 * 8 * ns * 9 * sizeof(cx<R>));
   const int blk = xcd_block(g.nxcd, g.rev);
   const int i = blk * ns + il;
-  stage_links<P>(lu, lnb, a.U, g, blk, ns, 2, (1 + a.cb_in) & 1);
+  constexpr bool kPQ = std::is_same<P, PI16>::value;  // integer hop
+  if constexpr (kPQ)
+    stage_links_pq(reinterpret_cast<LinkPQ*>(lu), lnb, a.U, g, blk, ns, 2, (1 + a.cb_in) & 1);
+  else
+    stage_links<P>(lu, lnb, a.U, g, blk, ns, 2, (1 + a.cb_in) & 1);
   __syncthreads();
   const bool act = j < a.nrhs && a.active[j] && i < g.vh;
   const double a_cg = (CG && j < a.nrhs) ? cg_alpha(a.scal + (size_t)j * MR_NSCAL) : 0.0;
@@ -2039,8 +2217,13 @@ __global__ void __launch_bounds__(512) k_dwf_mrhs(MrArgs<P, PX> a) {
     cx<R> acc[12];
 #pragma unroll
     for (int c = 0; c < 12; ++c) acc[c] = cx<R>{0, 0};
-    hop4<DAG>(acc, PlainLoader<P>{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh},
-              lu + ((s & 1) * 8 * ns + il) * 9, ns * 9, lnb + (s & 1) * 8 * ns + il, ns);
+    const PlainLoader<P> ld{in.d + s * slice, in.sc + (size_t)s * g.vh, g.vh};
+    if constexpr (kPQ)
+      hop4_pq<DAG>(acc, ld, reinterpret_cast<const LinkPQ*>(lu) + ((s & 1) * 8 * ns + il) * 9,
+                   ns * 9, lnb + (s & 1) * 8 * ns + il, ns);
+    else
+      hop4<DAG>(acc, ld, lu + ((s & 1) * 8 * ns + il) * 9, ns * 9, lnb + (s & 1) * 8 * ns + il,
+                ns);
     const int sm = s > 0 ? s - 1 : g.ls - 1, spl = s < g.ls - 1 ? s + 1 : 0;
     const double nm = -2.0 * a.mass[j] * (0.5 * a.two_a5);
     const R cm = (R)(s > 0 ? a.two_a5 : nm);
