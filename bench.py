@@ -274,6 +274,18 @@ def main():
     ap.add_argument("--no-learned", action="store_true",
                     help="skip training the network preconditioner and its solve")
     ap.add_argument("--learned-sloppy", default="i16", choices=["f32", "bf16", "i16"])
+    # training recipe of the network: `solves` Gaussian sources, each giving
+    # `pairs-per-solve` pairs from its own CG iterates every `stride` steps
+    # (e = x* - x_k, r = A e) plus `random` pairs with Gaussian e.  The
+    # default is the quick high-mode recipe; 20 x 100 solver pairs, random 0,
+    # is the 2000-pair recipe of the learned-preconditioner workload.
+    ap.add_argument("--learned-solves", type=int, default=6)
+    ap.add_argument("--learned-pairs-per-solve", type=int, default=0)
+    ap.add_argument("--learned-random", type=int, default=32)
+    ap.add_argument("--learned-stride", type=int, default=3)
+    ap.add_argument("--learned-epochs", type=int, default=3)
+    ap.add_argument("--gcr-krylov", type=int, default=8,
+                    help="restart length of the flexible GCR solve with the network (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); multi-GPU "
@@ -374,8 +386,11 @@ def main():
             log("[bench] training the network preconditioner")
             op.ptc_init(5, 0.02)
             t1 = time.perf_counter()
-            l0, l1, npairs = op.ptc_train(mass, nsolves=6, pairs_per_solve=0, random_pairs=32,
-                                          stride=3, epochs=3, batch=4, lr=2e-3, seed=17)
+            l0, l1, npairs = op.ptc_train(mass, nsolves=args.learned_solves,
+                                          pairs_per_solve=args.learned_pairs_per_solve,
+                                          random_pairs=args.learned_random,
+                                          stride=args.learned_stride,
+                                          epochs=args.learned_epochs, batch=4, lr=2e-3, seed=17)
             t_train = time.perf_counter() - t1
             vsol.upload(np.zeros(op.n))
             op.InvCgPtc_dev(vsol, vin, warm, args.learned_sloppy, args.delta)
@@ -397,8 +412,32 @@ def main():
                 f"true res {tr:.3e}, {ts:.2f}s ({t_plain / ts:.2f}x)")
             if st == 0 and tr <= 1e-10 and (best is None or ts < best[1]):
                 best = ("learned", ts)
+            solve["learned_fcg"]["training"]["recipe"] = {
+                "solves": args.learned_solves, "pairs_per_solve": args.learned_pairs_per_solve,
+                "random_pairs": args.learned_random, "stride": args.learned_stride,
+                "epochs": args.learned_epochs}
+            if args.gcr_krylov > 0:
+                # flexible GCR(m) with the same network, FP64 vectors and operator
+                vsol.upload(np.zeros(op.n))
+                op.InvGcrPtc_dev(vsol, vin, warm, args.gcr_krylov)
+                vsol.upload(np.zeros(op.n))
+                t1 = time.perf_counter()
+                it, tr, st, nnet, ncyc = op.InvGcrPtc_dev(vsol, vin, arg, args.gcr_krylov)
+                ts = time.perf_counter() - t1
+                err = float(np.linalg.norm(vsol.download() - x_plain)) / nx
+                solve["learned_fgcr"] = {"seconds": round(ts, 4), "iters": it, "net_steps": nnet,
+                                         "restarts": ncyc, "krylov": args.gcr_krylov,
+                                         "true_res": tr, "status": st,
+                                         "rel_err_vs_plain_fp64": err,
+                                         "speedup_vs_plain_fp64": round(t_plain / ts, 3)}
+                log(f"[bench] network: flexible GCR({args.gcr_krylov}) {it} iters ({nnet} with "
+                    f"the net, {ncyc} restarts), true res {tr:.3e}, {ts:.2f}s "
+                    f"({t_plain / ts:.2f}x)")
+                if st == 0 and tr <= 1e-10 and (best is None or ts < best[1]):
+                    best = ("learned_fgcr", ts)
         if best:
-            name = "learned_fcg" if best[0] == "learned" else f"mixed_{best[0]}_cg"
+            name = ("learned_fcg" if best[0] == "learned" else
+                    best[0] if best[0] == "learned_fgcr" else f"mixed_{best[0]}_cg")
             solve["best"] = {"method": name,
                              "speedup_vs_plain_fp64": round(t_plain / best[1], 3)}
         vsol.free()

@@ -2436,6 +2436,118 @@ int eig_deflate(dwfgpu_s* h, EigWork& w, const DVec& e, const double* U, int def
   return eig_mcombo(h, w, (double*)e.d, U, h->n, def_len, c);
 }
 
+// Flexible GCR(m) with the network as preconditioner (no CPS counterpart;
+// InvCg's stop rule and FP64 true residual judge it).  Per step: z = N(r),
+// w = A z made orthogonal to the cycle's earlier w_j by classical
+// Gram-Schmidt with complex coefficients c_j = <w_j, w> / |w_j|^2 (the
+// network is not Hermitian), p = z updated with the same coefficients, then
+// x += alpha p, r -= alpha w with alpha = <w, r> / |w|^2, which minimises
+// |r| over the cycle's space; the cycle restarts after m directions.  All of
+// it in FP64 with the FP64 operator.  A step that cuts |r|^2 by less than
+// 0.1 % counts against the network; after 10 such steps, or without a
+// network, plain CG (InvCg) finishes from the current solution.  The end is
+// always the true residual |src - A sol|.
+int invgcr_ptc(dwfgpu_s* h, const dwfgpu_cg_arg* arg, int m, const DVec& sol, const DVec& src,
+               double* true_res, int* iters, int* net_steps, int* restarts) {
+  typedef Ops<P64> H;
+  TRY(ensure_work<P64>(h, 6));  // work[0..2] stay free for the CG that may finish
+  const DVec& r = h->f[DWFGPU_F64].work[3];
+  const DVec& z = h->f[DWFGPU_F64].work[4];
+  const DVec& w = h->f[DWFGPU_F64].work[5];
+  const size_t n = h->n;
+  const double mass = arg->mass;
+  struct Dirs {  // p_j and w_j = A p_j of the cycle, m vectors each
+    double* p = nullptr;
+    double* w = nullptr;
+    ~Dirs() {
+      if (p) (void)hipFree(p);
+      if (w) (void)hipFree(w);
+    }
+  } B;
+  HIPCHK(hipMalloc(&B.p, (size_t)m * n * sizeof(double)));
+  HIPCHK(hipMalloc(&B.w, (size_t)m * n * sizeof(double)));
+  EigWork ew;
+  TRY(ew.init(1));
+  double sn;
+  TRY(H::norm(h, src, &sn));
+  const double stp = sn * arg->stop_rsd * arg->stop_rsd;
+  auto true_residual = [&](double* rr) -> int {  // r = src - A sol, |r|^2
+    TRY(H::mdagm(h, mass, w, sol, nullptr));
+    TRY(H::axpy_to(h, r, w, src, -1.0, true));
+    return fetch_sum(h, dwfk::BLAS_BLOCKS, rr);
+  };
+  double rr;
+  TRY(true_residual(&rr));
+  std::vector<double> wn(m, 0.0);
+  int itr = 0, k = 0, nsteps = 0, nbad = 0, ncycle = 0, status = DWFGPU_OK;
+  const int max_itr = arg->max_num_iter - 1;
+  bool use_net = h->net != nullptr;
+  while (use_net && rr > stp && itr < max_itr) {
+    ++itr;
+    TRY(ptc_apply<P64>(h, z, r));
+    double zz;
+    TRY(H::norm(h, z, &zz));
+    if (!std::isfinite(zz) || zz == 0.0) TRY(H::copy(h, z, r));
+    TRY(H::mdagm(h, mass, w, z, nullptr));
+    double* pk = B.p + (size_t)k * n;
+    double* wk = B.w + (size_t)k * n;
+    HIPCHK(hipMemcpyAsync(pk, z.d, n * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipMemcpyAsync(wk, w.d, n * sizeof(double), hipMemcpyDeviceToDevice, h->st));
+    if (k > 0) {
+      std::vector<std::complex<double>> c;
+      TRY(eig_mdot(h, ew, B.w, n, k, wk, c));
+      for (int j = 0; j < k; ++j) c[j] = -c[j] / wn[j];
+      TRY(eig_mcombo(h, ew, wk, B.w, n, k, c));
+      TRY(eig_mcombo(h, ew, pk, B.p, n, k, c));
+    }
+    DVec wkv;
+    wkv.d = wk;
+    TRY(H::norm(h, wkv, &wn[k]));
+    if (!(wn[k] > 0.0) || !std::isfinite(wn[k])) {
+      status = DWFGPU_BREAKDOWN;
+      break;
+    }
+    std::vector<std::complex<double>> a;
+    TRY(eig_mdot(h, ew, wk, n, 1, (const double*)r.d, a));
+    const std::complex<double> alpha = a[0] / wn[k];
+    TRY(eig_mcombo(h, ew, (double*)sol.d, pk, n, 1, {alpha}));
+    TRY(eig_mcombo(h, ew, (double*)r.d, wk, n, 1, {-alpha}));
+    double rrn;
+    TRY(H::norm(h, r, &rrn));
+    if (rrn > (1.0 - 1e-3) * rr) {
+      if (++nbad >= 10) use_net = false;
+    } else {
+      ++nsteps;
+    }
+    rr = rrn;
+    if (++k == m) {
+      k = 0;
+      ++ncycle;
+    }
+  }
+  if (status == DWFGPU_OK) {
+    TRY(true_residual(&rr));
+    if (rr > stp && itr < max_itr) {
+      // plain CG from the current solution (the fallback)
+      dwfgpu_cg_arg a2 = *arg;
+      a2.max_num_iter = arg->max_num_iter - itr;
+      double tr2 = 0.0;
+      int it2 = 0;
+      status = H::invcg(h, &a2, sol, src, &tr2, &it2);
+      if (status != DWFGPU_OK && status != DWFGPU_MAXITER) return status;
+      itr += it2 - 1;
+      rr = tr2 * tr2 * sn;
+    } else if (rr > stp) {
+      status = DWFGPU_MAXITER;
+    }
+  }
+  if (true_res) *true_res = std::sqrt(rr / sn);
+  if (iters) *iters = itr + 1;
+  if (net_steps) *net_steps = nsteps;
+  if (restarts) *restarts = ncycle;
+  return status;
+}
+
 int inveigcg_run(dwfgpu_s* h, const dwfgpu_cg_arg* arg, const DVec& sol, const DVec& src,
                  int nev, int m, double* V, double* M, const double* U,
                  const std::vector<std::complex<double>>& invH, int def_len,
@@ -3853,6 +3965,20 @@ int dwfgpu_invcg_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int sloppy, doubl
       set_err("dwfgpu_invcg_ptc_dev: sloppy precision must be F32, BF16 or I16");
       return DWFGPU_BAD_ARG;
   }
+}
+
+int dwfgpu_invgcr_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg* arg, int nkrylov, dwfgpu_vec_t sol,
+                          dwfgpu_vec_t src, double* true_res, int* iters, int* net_steps,
+                          int* restarts) {
+  if (!h || !arg || !sol || !src || sol->h != h || src->h != h) return DWFGPU_BAD_ARG;
+  if (h->prec != DWFGPU_F64 || arg->max_num_iter < 1 || nkrylov < 1 || nkrylov > 64) {
+    set_err("dwfgpu_invgcr_ptc_dev: FP64 handle, max_num_iter >= 1 and 1 <= nkrylov <= 64 "
+            "needed");
+    return DWFGPU_BAD_ARG;
+  }
+  const int st = invgcr_ptc(h, arg, nkrylov, sol->v, src->v, true_res, iters, net_steps, restarts);
+  (void)hipStreamSynchronize(h->st);
+  return st;
 }
 
 int dwfgpu_minvcg_dev(dwfgpu_t h, double mass, const double* shifts, int nshift,

@@ -20,8 +20,13 @@
 // weight gradient is the contraction the other way round,
 //   P_d[M' x K'] = sum_{sites, colours} X[M' x N] Y_d[K' x N]^T
 // with X the output gradient and Y_d the transported input.
-// MFMA: v_mfma_f32_16x16x16_bf16 (lane l holds A[l%16][4(l/16)..+3],
-// B[4(l/16)..+3][l%16] and C[4(l/16)..+3][l%16]).
+// MFMA: v_mfma_f32_16x16x32_bf16 (lane l holds A[l%16][8(l/16)..+7],
+// B[8(l/16)..+7][l%16] and C[4(l/16)..+3][l%16]); a K remainder of 16 takes
+// one v_mfma_f32_16x16x16_bf16 (4 elements per lane).  The four waves of a
+// work-group split the output blocks as a 2 x 2 grid: each wave keeps the A
+// fragments of its block rows and the B fragments of its block columns in
+// registers for one K step and issues every product between them, so each
+// fragment is read from LDS once per step, not once per output block.
 #pragma once
 #include "dwf_kernels.hpp"
 
@@ -41,6 +46,7 @@ constexpr int PADK = 8;         // LDS row padding of [col][K'] tiles (bf16 elem
 constexpr int ROWC = NCOL + 4;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float x) { return (uint16_t)dwfk::bf16_rne(x); }
@@ -236,6 +242,15 @@ __device__ __forceinline__ bf16x4 ld4(const uint16_t* p) {
   r[3] = (short)(w.y >> 16);
   return r;
 }
+// 8 consecutive bf16 (16-byte aligned: one LDS load)
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+// 8 consecutive bf16 at an 8-byte boundary (two LDS loads)
+__device__ __forceinline__ bf16x8 ld8x2(const uint16_t* p) {
+  const uint2 a = *reinterpret_cast<const uint2*>(p), b = *reinterpret_cast<const uint2*>(p + 4);
+  return __builtin_bit_cast(bf16x8, uint4{a.x, a.y, b.x, b.y});
+}
 
 // LDS bytes of the forward kernel
 template <int CIN, int COUT>
@@ -255,8 +270,11 @@ __global__ void __launch_bounds__(NTHR)
 k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
           const uint16_t* __restrict__ A, const typename dwfk::P32::S* __restrict__ U, Geom g) {
   constexpr int MP = 8 * COUT, KP = 8 * CIN;
-  constexpr int MB = MP / 16, NB = NCOL / 16, KB = KP / 16;
-  constexpr int NBLK = MB * NB, PER = (NBLK + 3) / 4;
+  constexpr int MB = MP / 16, NB = NCOL / 16;
+  constexpr int K32 = KP / 32, K16 = (KP % 32) / 16;  // K steps of 32, a tail of 16
+  static_assert(KP % 16 == 0 && NB % 2 == 0 && ((KP + PADK) * 2) % 16 == 0, "tile shapes");
+  // this wave's output blocks: rows [mb0, mb0 + RM) x columns [nb0, nb0 + RN)
+  constexpr int RM = (MB + 1) / 2, RN = NB / 2;
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t* sA = reinterpret_cast<uint16_t*>(smem);
   uint16_t* sB = sA + MP * (KP + PADK);
@@ -271,9 +289,12 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
   dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par);
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  f32x4 acc[PER];
+  const int mb0 = (wave & 1) * RM, nb0 = (wave >> 1) * RN;
+  f32x4 acc[RM][RN];
 #pragma unroll
-  for (int b = 0; b < PER; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < RM; ++i)
+#pragma unroll
+    for (int j = 0; j < RN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
   for (int d = 0; d < NDIR; ++d) {
     __syncthreads();  // links staged / previous direction's operands consumed
@@ -303,17 +324,36 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
     }
     build_transported<CIN, false>(sB, in, lu, lnb, d, par, i0, g);
     __syncthreads();
+    const uint16_t* pa = sA + (mb0 * 16 + (lane & 15)) * (KP + PADK);
+    const uint16_t* pb = sB + (nb0 * 16 + (lane & 15)) * (KP + PADK);
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      const int k0 = kb * 16 + 4 * (lane >> 4);
+    for (int kb = 0; kb < K32; ++kb) {
+      const int k0 = kb * 32 + 8 * (lane >> 4);
+      bf16x8 bfr[RN];
 #pragma unroll
-      for (int b = 0; b < PER; ++b) {
-        const int blk = wave + 4 * b;
-        if (blk < NBLK) {
-          const int mb = blk / NB, nb = blk - mb * NB;
-          const bf16x4 af = ld4(sA + (mb * 16 + (lane & 15)) * (KP + PADK) + k0);
-          const bf16x4 bfr = ld4(sB + (nb * 16 + (lane & 15)) * (KP + PADK) + k0);
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr, acc[b], 0, 0, 0);
+      for (int j = 0; j < RN; ++j) bfr[j] = ld8(pb + j * 16 * (KP + PADK) + k0);
+#pragma unroll
+      for (int i = 0; i < RM; ++i) {
+        if (mb0 + i < MB) {
+          const bf16x8 af = ld8(pa + i * 16 * (KP + PADK) + k0);
+#pragma unroll
+          for (int j = 0; j < RN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[j], acc[i][j], 0, 0, 0);
+        }
+      }
+    }
+    if constexpr (K16 > 0) {
+      const int k0 = K32 * 32 + 4 * (lane >> 4);
+      bf16x4 bfr[RN];
+#pragma unroll
+      for (int j = 0; j < RN; ++j) bfr[j] = ld4(pb + j * 16 * (KP + PADK) + k0);
+#pragma unroll
+      for (int i = 0; i < RM; ++i) {
+        if (mb0 + i < MB) {
+          const bf16x4 af = ld4(pa + i * 16 * (KP + PADK) + k0);
+#pragma unroll
+          for (int j = 0; j < RN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr[j], acc[i][j], 0, 0, 0);
         }
       }
     }
@@ -321,13 +361,14 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
   __syncthreads();
   // C (lane: rows 4(l/16)..+3 of column l%16) -> LDS [M'][NCOL]
 #pragma unroll
-  for (int b = 0; b < PER; ++b) {
-    const int blk = wave + 4 * b;
-    if (blk < NBLK) {
-      const int mb = blk / NB, nb = blk - mb * NB;
+  for (int i = 0; i < RM; ++i) {
+    if (mb0 + i < MB) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        sC[(mb * 16 + 4 * (lane >> 4) + r) * NCOL + nb * 16 + (lane & 15)] = acc[b][r];
+      for (int j = 0; j < RN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          sC[((mb0 + i) * 16 + 4 * (lane >> 4) + r) * NCOL + (nb0 + j) * 16 + (lane & 15)] =
+              acc[i][j][r];
     }
   }
   __syncthreads();
@@ -355,8 +396,10 @@ __global__ void __launch_bounds__(NTHR)
 k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
          const uint32_t* __restrict__ in, const typename dwfk::P32::S* __restrict__ U, Geom g) {
   constexpr int MP = 8 * COUT, KP = 8 * CIN;
-  constexpr int MB = MP / 16, KB2 = KP / 16, CB = NCOL / 16;
-  constexpr int NBLK = MB * KB2, PER = (NBLK + 3) / 4;
+  constexpr int MB = MP / 16, KB2 = KP / 16, CB = NCOL / 32;  // contraction over columns
+  static_assert(NCOL % 32 == 0, "column steps of 32");
+  // this wave's output blocks: rows [mb0, mb0 + RM) x columns [kb0, kb0 + RK)
+  constexpr int RM = (MB + 1) / 2, RK = (KB2 + 1) / 2;
   extern __shared__ __align__(16) unsigned char smem[];
   uint16_t* sX = reinterpret_cast<uint16_t*>(smem);
   uint16_t* sY = sX + MP * ROWC;
@@ -365,9 +408,12 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
   const int d = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tiles_per_par = g.vh / NT, ntiles = 2 * tiles_per_par;
-  f32x4 acc[PER];
+  const int mb0 = (wave & 1) * RM, kb0 = (wave >> 1) * RK;
+  f32x4 acc[RM][RK];
 #pragma unroll
-  for (int b = 0; b < PER; ++b) acc[b] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < RM; ++i)
+#pragma unroll
+    for (int j = 0; j < RK; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int par = t / tiles_per_par;
     const int i0 = (t - par * tiles_per_par) * NT;
@@ -404,30 +450,39 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
     __syncthreads();  // links staged
     build_transported<CIN, true>(sY, in, lu, lnb, d, par, i0, g);
     __syncthreads();
+    const uint16_t* px = sX + (mb0 * 16 + (lane & 15)) * ROWC;
+    const uint16_t* py = sY + (kb0 * 16 + (lane & 15)) * ROWC;
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-      const int c0 = cb * 16 + 4 * (lane >> 4);
+      // rows of ROWC = 100 bf16 are 8-byte aligned: two loads per fragment
+      const int c0 = cb * 32 + 8 * (lane >> 4);
+      bf16x8 yf[RK];
 #pragma unroll
-      for (int b = 0; b < PER; ++b) {
-        const int blk = wave + 4 * b;
-        if (blk < NBLK) {
-          const int mb = blk / KB2, kb = blk - mb * KB2;
-          const bf16x4 xf = ld4(sX + (mb * 16 + (lane & 15)) * ROWC + c0);
-          const bf16x4 yf = ld4(sY + (kb * 16 + (lane & 15)) * ROWC + c0);
-          acc[b] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(xf, yf, acc[b], 0, 0, 0);
+      for (int j = 0; j < RK; ++j)
+        if (kb0 + j < KB2) yf[j] = ld8x2(py + j * 16 * ROWC + c0);
+#pragma unroll
+      for (int i = 0; i < RM; ++i) {
+        if (mb0 + i < MB) {
+          const bf16x8 xf = ld8x2(px + i * 16 * ROWC + c0);
+#pragma unroll
+          for (int j = 0; j < RK; ++j)
+            if (kb0 + j < KB2)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xf, yf[j], acc[i][j], 0, 0, 0);
         }
       }
     }
   }
   float* dst = part + ((size_t)d * gridDim.x + blockIdx.x) * MP * KP;
 #pragma unroll
-  for (int b = 0; b < PER; ++b) {
-    const int blk = wave + 4 * b;
-    if (blk < NBLK) {
-      const int mb = blk / KB2, kb = blk - mb * KB2;
+  for (int i = 0; i < RM; ++i) {
+    if (mb0 + i < MB) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        dst[(mb * 16 + 4 * (lane >> 4) + r) * KP + kb * 16 + (lane & 15)] = acc[b][r];
+      for (int j = 0; j < RK; ++j)
+        if (kb0 + j < KB2)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            dst[((mb0 + i) * 16 + 4 * (lane >> 4) + r) * KP + (kb0 + j) * 16 + (lane & 15)] =
+                acc[i][j][r];
     }
   }
 }

@@ -48,7 +48,7 @@ EXPORTS = (
     "dwfgpu_matinv_mrhs_dev",
     "dwfgpu_ptc_num_weights", "dwfgpu_ptc_init", "dwfgpu_ptc_set_weights",
     "dwfgpu_ptc_get_weights", "dwfgpu_ptc_apply", "dwfgpu_ptc_loss_grad", "dwfgpu_ptc_train",
-    "dwfgpu_invcg_ptc_dev", "dwfgpu_minvcg", "dwfgpu_minvcg_dev",
+    "dwfgpu_invcg_ptc_dev", "dwfgpu_invgcr_ptc_dev", "dwfgpu_minvcg", "dwfgpu_minvcg_dev",
     "dwfgpu_time_matpcdagmatpc", "dwfgpu_matpcdagmatpc_cost", "dwfgpu_time_copy",
     "dwfgpu_time_bandwidth",
     "dwfgpu_fermion_from_canonical", "dwfgpu_fermion_to_canonical", "dwfgpu_four2five",
@@ -177,6 +177,9 @@ def lib():
         L.dwfgpu_invcg_ptc_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, C.c_double, H, H,
                                            C.POINTER(C.c_double), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.dwfgpu_invgcr_ptc_dev.argtypes = [H, C.POINTER(CgArgC), C.c_int, H, H,
+                                            C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.dwfgpu_time_matpcdagmatpc.argtypes = [H, C.c_int, C.c_double, H, H, C.c_int,
                                                 C.POINTER(C.c_double), _dp, _dp,
                                                 C.POINTER(C.c_int)]
@@ -504,6 +507,17 @@ class _DiracOpGpu:
                                              src.h, C.byref(tr), C.byref(it), C.byref(nr),
                                              C.byref(ns)), "InvCgPtc_dev", allow=(OK, MAXITER))
         return it.value, tr.value, st, nr.value, ns.value
+
+    def InvGcrPtc_dev(self, sol: DeviceVector, src: DeviceVector, cg_arg: CgArg, nkrylov=8):
+        """Flexible GCR(nkrylov) with the network as preconditioner (plain CG
+        finishes if the network stops helping); returns
+        (iters, true_res, status, net_steps, restarts)."""
+        tr, it, ns, nc = C.c_double(), C.c_int(), C.c_int(), C.c_int()
+        a = cg_arg.c()
+        st = _chk(lib().dwfgpu_invgcr_ptc_dev(self.h, C.byref(a), nkrylov, sol.h, src.h,
+                                              C.byref(tr), C.byref(it), C.byref(ns),
+                                              C.byref(nc)), "InvGcrPtc_dev", allow=(OK, MAXITER))
+        return it.value, tr.value, st, ns.value, nc.value
 
     def MatInvMrhs_dev(self, out_first: MultiVector, out_second: MultiVector,
                        in_first: MultiVector, in_second: MultiVector, cg_arg: CgArg, masses,

@@ -276,6 +276,18 @@ int dwfgpu_ptc_train(dwfgpu_t h, const dwfgpu_ptc_train_arg *a, double *loss_fir
 int dwfgpu_invcg_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, double delta,
                          dwfgpu_vec_t sol, dwfgpu_vec_t src, double *true_res, int *iters,
                          int *reliable_updates, int *net_steps);
+/* InvCg's solve (same stop rule on the FP64 true residual) by flexible GCR
+ * with the network as preconditioner, restarted every nkrylov (1..64)
+ * directions: each step z = N(r), w = A z made orthogonal to the cycle's
+ * earlier w_j (complex Gram-Schmidt; the network is not Hermitian), x and r
+ * updated to minimise |r| over the cycle's space; FP64 vectors and operator.
+ * After 10 steps that cut |r|^2 by less than 0.1 %, or without a network,
+ * plain CG (dwfgpu_invcg) finishes from the current solution; *iters counts
+ * both (CPS's convention, steps + 1), *net_steps the steps the network
+ * helped, *restarts the completed cycles.  No CPS counterpart.            */
+int dwfgpu_invgcr_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int nkrylov, dwfgpu_vec_t sol,
+                          dwfgpu_vec_t src, double *true_res, int *iters, int *net_steps,
+                          int *restarts);
 
 /* ---- instrumentation (the role of CPS's print_flops / CGflops) --------- */
 /* Times `reps` MatPcDagMatPc applications on the handle's stream with HIP

@@ -135,3 +135,29 @@ def test_training_and_preconditioned_solve():
     g.MatPcDagMatPc(ax, x, mass)
     assert rel(ax, src) <= 1.0001e-10
     assert ns1 > 0 and it1 < it0
+    # flexible GCR(8) with the same network: residual-minimising steps in FP64
+    vs = g.vector()
+    it2, tr2, st2, ns2, nc2 = g.InvGcrPtc_dev(vs, vb, arg, nkrylov=8)
+    say(f"  flexible GCR(8): iters={it2} (net helped on {ns2}, {nc2} restarts) "
+        f"true_res={tr2:.3e}")
+    assert st2 == 0 and tr2 <= 1e-10
+    ax = np.zeros(g.n)
+    g.MatPcDagMatPc(ax, vs.download(), mass)
+    assert rel(ax, src) <= 1.0001e-10
+    assert ns2 > 0 and it2 < it0
+
+
+def test_gcr_without_network_is_cg():
+    """No network: the GCR entry hands the whole solve to InvCg (same
+    iterations and true residual as dwfgpu_invcg_dev)."""
+    dims, ls, mass = (4, 4, 4, 8), 4, 0.1
+    say("[gpu] flexible GCR without a network")
+    U, g = make("mobius", dims, ls)
+    src = lat.to_reals(lat.gaussian_spinor((ls, int(np.prod(dims)) // 2), 41))
+    arg = CgArg(mass, 5000, 1e-10)
+    vs, vb = g.vector(), g.vector(src)
+    it, tr, st, ns, nc = g.InvGcrPtc_dev(vs, vb, arg, nkrylov=4)
+    v2 = g.vector()
+    it_cg, tr_cg, st_cg = g.InvCg_dev(v2, vb, arg)
+    say(f"  gcr entry {it} iters, true res {tr:.3e}; InvCg {it_cg} iters")
+    assert st == 0 and ns == 0 and it == it_cg and abs(tr - tr_cg) <= 1e-3 * tr_cg
