@@ -17,6 +17,8 @@ against the CPU oracle at the full lattice sizes the solvers are run at.
     batched FP64 operator against the oracle for all 24 RHS, the batched FP32
     solve with every RHS's residual checked by the oracle operator, the
     single-RHS count within 2 per RHS, and one RHS against an oracle solve.
+  * the batched path of a Mobius handle at the same size (24 RHS): operator
+    against the oracle, batched FP32 CG residuals by the oracle operator.
   * the network preconditioner at 16^3x32, Ls 16 (Mobius): forward and weight
     gradients against oracle/ptc_net.py within the BF16 bounds of
     tests/test_gpu_ptc.py.
@@ -220,6 +222,42 @@ def test_dwf_16x16x16x32_ls16_mrhs24_vs_oracle():
     err = rel(sol.download(j), xo)
     say(f"  RHS {j}: batched {it[j]} iterations, oracle {oit}; |x - x_oracle|/|x_oracle| {err:.2e}")
     assert err < 1e-8 and abs(int(it[j]) - oit) <= 3
+
+
+@pytest.mark.timeout(900)
+def test_mobius_16x16x16x32_ls16_mrhs24_vs_oracle():
+    """The batched path of Mobius handles at the 24-RHS workload's size: all
+    24 operator applications against the oracle (and bit-equal to the
+    single-RHS operator for a few), the batched FP32 CG with every RHS's
+    residual checked by the oracle operator."""
+    dims, ls = (16, 16, 16, 32), 16
+    masses = np.array([(0.005, 0.01, 0.04)[j % 3] for j in range(24)])
+    say(f"[gpu] 24-RHS batched Mobius {dims} ls={ls}: operator and solve vs oracle")
+    U = lat.random_gauge(dims, 0.3, 1234)
+    o = Oracle(dims, ls, "mobius", m5=M5, b=1.5, c=0.5, gauge_canonical=U)
+    g = DiracOpMobiusGpu(U, dims, ls, m5=M5, b=1.5, c=0.5)
+    srcs = [src_vec(dims, ls, 300 + j) for j in range(24)]
+    vin, vout = g.multi_vector(24, srcs), g.multi_vector(24)
+    g.MatPcDagMatPc_mrhs_dev(vout, vin, masses, "f64")
+    worst = 0.0
+    single = np.zeros(g.n)
+    for j in range(24):
+        got = vout.download(j)
+        want, _ = o.matpcdagmatpc(srcs[j], masses[j])
+        worst = max(worst, site_rel(got, want))
+        if j in (0, 13, 23):
+            g.MatPcDagMatPc(single, srcs[j], masses[j])
+            assert np.array_equal(got, single), j
+    say(f"  batched FP64 operator, 24 RHS: max site rel err {worst:.2e}")
+    assert worst < 1e-13
+    sol = g.multi_vector(24)
+    it, tr, st, nrel = g.InvCgMrhs_dev(sol, vin, CgArg(0.0, 5000, 1e-10), masses, "f32", 0.1)
+    say(f"  batched f32 CG iterations {it.tolist()}")
+    assert np.all(st == 0) and np.all(tr <= 1e-10)
+    for j in range(24):
+        ax, _ = o.matpcdagmatpc(sol.download(j), masses[j])
+        otrue = float(np.linalg.norm(srcs[j] - ax) / np.linalg.norm(srcs[j]))
+        assert otrue <= 1.0001e-10, (j, otrue)
 
 
 HID = 16
