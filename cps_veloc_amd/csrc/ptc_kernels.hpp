@@ -51,9 +51,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ uint16_t f2bf(float x) { return (uint16_t)dwfk::bf16_rne(x); }
 __device__ __forceinline__ float bf2f(uint32_t b16) { return __uint_as_float(b16 << 16); }
-__device__ __forceinline__ uint32_t pack_c(cx<float> v) {
-  return (uint32_t)f2bf(v.re) | ((uint32_t)f2bf(v.im) << 16);
-}
+__device__ __forceinline__ uint32_t pack_c(cx<float> v) { return dwfk::bf16_pair(v.re, v.im); }
 __device__ __forceinline__ cx<float> unpack_c(uint32_t w) {
   return {__uint_as_float(w << 16), __uint_as_float(w & 0xffff0000u)};
 }
