@@ -1153,7 +1153,7 @@ struct PtcNet {
   uint32_t* act[4] = {nullptr, nullptr, nullptr, nullptr};
   uint32_t* gact[2] = {nullptr, nullptr};
   float* gwpart = nullptr;
-  int nwg = 64;
+  int nwg[3] = {0, 0, 0};  // work-groups per direction of each layer's weight gradient
   cx<float>* G[3] = {nullptr, nullptr, nullptr};
   int cin(int k) const { return k == 0 ? ls : hid; }
   int cout(int k) const { return k == 2 ? ls : hid; }
@@ -1214,14 +1214,30 @@ void ptc_real_form(const std::vector<std::complex<float>>& W, int cin, int cout,
   }
 }
 
+// [9][M'][K'] row-major -> the MFMA fragment order of k_ptc_fwd: 8 bf16
+// A[d][16*blk + l%16][32*kb + 8*(l/16) + e] at (((d*MB + blk)*K32 + kb)*64 + l)*8 + e
+void ptc_fragment_order(std::vector<uint16_t>& buf, int mp, int kp) {
+  const int mb = mp / 16, k32 = kp / 32;
+  std::vector<uint16_t> rm(buf);
+  for (int d = 0; d < ptck::NDIR; ++d)
+    for (int blk = 0; blk < mb; ++blk)
+      for (int kb = 0; kb < k32; ++kb)
+        for (int l = 0; l < 64; ++l)
+          for (int e = 0; e < 8; ++e)
+            buf[((((size_t)d * mb + blk) * k32 + kb) * 64 + l) * 8 + e] =
+                rm[(size_t)d * mp * kp + (size_t)(16 * blk + l % 16) * kp + 32 * kb + 8 * (l / 16) + e];
+}
+
 int ptc_upload(dwfgpu_s* h) {
   PtcNet* n = h->net;
   std::vector<uint16_t> buf;
   for (int k = 0; k < 3; ++k) {
     ptc_real_form(n->W[k], n->cin(k), n->cout(k), false, buf);
+    ptc_fragment_order(buf, 8 * n->cout(k), 8 * n->cin(k));
     HIPCHK(hipMemcpyAsync(n->A[k], buf.data(), buf.size() * 2, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     ptc_real_form(n->W[k], n->cin(k), n->cout(k), true, buf);
+    ptc_fragment_order(buf, 8 * n->cin(k), 8 * n->cout(k));
     HIPCHK(hipMemcpyAsync(n->Aadj[k], buf.data(), buf.size() * 2, hipMemcpyHostToDevice, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
   }
@@ -1236,8 +1252,25 @@ int ptc_fwd_t(dwfgpu_s* h, uint32_t* out, const uint32_t* in, const uint16_t* A)
     return DWFGPU_BAD_ARG;
   }
   hipLaunchKernelGGL((ptck::k_ptc_fwd<CIN, COUT>), dim3(2 * h->g.vh / ptck::NT), dim3(ptck::NTHR),
-                     lds, h->st, out, in, A, (const typename P32::S*)h->f[DWFGPU_F32].U, h->g);
+                     lds, h->st, out, in, reinterpret_cast<const ptck::u32x4*>(A),
+                     (const typename P32::S*)h->f[DWFGPU_F32].U, h->g);
   HIPCHK(hipGetLastError());
+  return DWFGPU_OK;
+}
+
+// Work-groups per direction of the weight-gradient kernel: as many as fill
+// every CU once (all NDIR directions resident together), so the loop over
+// tiles ends at the same time everywhere (576 work-groups over 512 resident
+// slots had left a second round of 64 with the device almost idle).
+template <int CIN, int COUT>
+int ptc_gw_grid(dwfgpu_s* h, int* nwg) {
+  constexpr size_t lds = ptck::gw_lds<CIN, COUT>();
+  int ncu = 0, per_cu = 0;
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
+  HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ptck::k_ptc_gw<CIN, COUT>,
+                                                      ptck::NTHR, lds));
+  *nwg = std::max(1, std::min(ncu * std::max(per_cu, 1) / ptck::NDIR,
+                              (int)(2 * h->g.vh / ptck::NT)));
   return DWFGPU_OK;
 }
 
@@ -1272,6 +1305,15 @@ int ptc_gw(dwfgpu_s* h, int cin, int cout, float* part, int nwg, const uint32_t*
   if (cin == CI && cout == CO) return ptc_gw_t<CI, CO>(h, part, nwg, gout, in);
   PTC_PAIRS(PTC_G)
 #undef PTC_G
+  set_err("ptc: no gradient kernel for %d -> %d channels", cin, cout);
+  return DWFGPU_BAD_ARG;
+}
+
+int ptc_gw_nwg(dwfgpu_s* h, int cin, int cout, int* nwg) {
+#define PTC_N(CI, CO) \
+  if (cin == CI && cout == CO) return ptc_gw_grid<CI, CO>(h, nwg);
+  PTC_PAIRS(PTC_N)
+#undef PTC_N
   set_err("ptc: no gradient kernel for %d -> %d channels", cin, cout);
   return DWFGPU_BAD_ARG;
 }
@@ -1323,9 +1365,9 @@ int ptc_grad_pair(dwfgpu_s* h, const DVec& r, const DVec& e, bool accumulate, do
   // backward: layer 2, 1, 0 (gact ping-pong)
   int cur = 0;
   for (int k = 2; k >= 0; --k) {
-    TRY(ptc_gw(h, n->cin(k), n->cout(k), n->gwpart, n->nwg, n->gact[cur], n->act[k]));
+    TRY(ptc_gw(h, n->cin(k), n->cout(k), n->gwpart, n->nwg[k], n->gact[cur], n->act[k]));
     hipLaunchKernelGGL(ptck::k_ptc_gw_reduce, dim3(128), dim3(256), 0, h->st, n->G[k], n->gwpart,
-                       n->nwg, n->cin(k), n->cout(k), accumulate ? 1 : 0);
+                       n->nwg[k], n->cin(k), n->cout(k), accumulate ? 1 : 0);
     HIPCHK(hipGetLastError());
     if (k > 0) {
       TRY(ptc_fwd(h, n->cout(k), n->cin(k), n->gact[1 - cur], n->gact[cur], n->Aadj[k]));
@@ -1392,9 +1434,14 @@ int ptc_ensure(dwfgpu_s* h) {
   const int chans[4] = {ls, n->hid, n->hid, ls};
   for (int k = 0; k < 4 && !st; ++k) st = alloc(h, (void**)&n->act[k], V * chans[k] * 12 * 4);
   for (int k = 0; k < 2 && !st; ++k) st = alloc(h, (void**)&n->gact[k], V * cmax * 12 * 4);
+  int nwg_max = 0;
+  for (int k = 0; k < 3 && !st; ++k) {
+    st = ptc_gw_nwg(h, n->cin(k), n->cout(k), &n->nwg[k]);
+    nwg_max = std::max(nwg_max, n->nwg[k]);
+  }
   if (!st)
     st = alloc(h, (void**)&n->gwpart,
-               (size_t)ptck::NDIR * n->nwg * 64 * n->hid * n->hid * sizeof(float));
+               (size_t)ptck::NDIR * nwg_max * 64 * n->hid * n->hid * sizeof(float));
   if (st) {
     ptc_free(n);
     h->net = nullptr;

@@ -711,7 +711,8 @@ template <class P>
 __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
                                             int* __restrict__ lnb,
                                             const typename P::S* __restrict__ U, const Geom& g,
-                                            int blk, int ns, int nsets, int p_first) {
+                                            int blk, int ns, int nsets, int p_first,
+                                            int only_dir = -1) {
   typedef typename P::R R;
   constexpr int NE = P::kLinkElems;
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
@@ -719,7 +720,7 @@ __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
   for (int w = tid; w < nsets * 8 * ns; w += nthr) {
     const int il = w % ns, d = (w / ns) % 8, set = w / (8 * ns);
     const int i = blk * ns + il;
-    if (i >= g.vh) continue;
+    if (i >= g.vh || (only_dir >= 0 && d != only_dir)) continue;
     const int p = (p_first + set) & 1;
     const int mu = d & 3;
     const int e = g.nbt[(size_t)(p * 8 + d) * g.vh + i];

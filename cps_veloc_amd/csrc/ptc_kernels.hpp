@@ -21,12 +21,16 @@
 //   P_d[M' x K'] = sum_{sites, colours} X[M' x N] Y_d[K' x N]^T
 // with X the output gradient and Y_d the transported input.
 // MFMA: v_mfma_f32_16x16x32_bf16 (lane l holds A[l%16][8(l/16)..+7],
-// B[8(l/16)..+7][l%16] and C[4(l/16)..+3][l%16]); a K remainder of 16 takes
-// one v_mfma_f32_16x16x16_bf16 (4 elements per lane).  The four waves of a
+// B[8(l/16)..+7][l%16] and C[4(l/16)..+3][l%16]).  The four waves of a
 // work-group split the output blocks as a 2 x 2 grid: each wave keeps the A
 // fragments of its block rows and the B fragments of its block columns in
 // registers for one K step and issues every product between them, so each
-// fragment is read from LDS once per step, not once per output block.
+// fragment is read once per step, not once per output block.  The forward
+// layer reads its weight fragments straight from memory (pre-arranged in
+// fragment order) and only the transported field goes through LDS.
+// Transport: the links are staged as packed bf16 pairs so that each complex
+// multiply-add of U phi is two v_dot2_f32_bf16 on the activation words as
+// stored (FP32 accumulation).
 #pragma once
 #include "dwf_kernels.hpp"
 
@@ -45,11 +49,10 @@ constexpr int PADK = 8;         // LDS row padding of [col][K'] tiles (bf16 elem
 // a tile land in different LDS banks (a stride of 104 repeats every 16 rows)
 constexpr int ROWC = NCOL + 4;
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float x) { return (uint16_t)dwfk::bf16_rne(x); }
 __device__ __forceinline__ float bf2f(uint32_t b16) { return __uint_as_float(b16 << 16); }
 __device__ __forceinline__ uint32_t pack_c(cx<float> v) { return dwfk::bf16_pair(v.re, v.im); }
 __device__ __forceinline__ cx<float> unpack_c(uint32_t w) {
@@ -139,22 +142,79 @@ k_ptc_loss(uint32_t* __restrict__ gact, const uint32_t* __restrict__ a3,
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// transported colour vector: d = 0 identity, 1..4 U phi, 5..8 U^dag phi
-__device__ __forceinline__ void transport3(const cx<float>* u, int d, const cx<float>* f,
-                                           cx<float>* v) {
+// The tile's links as packed bf16 words for v_dot2_f32_bf16.  For the
+// transport matrix M of network direction d (U_mu(x) for d = 1..4, U_mu(x-mu)^dag
+// for d = 5..8), entry (a, b) becomes two words, R = (Re M_ab, -Im M_ab) and
+// I = (Im M_ab, Re M_ab) (low half first), so that for a colour vector f held as
+// activations (f_b = (Re f_b, Im f_b)):
+//   Re (M f)_a = sum_b dot2(R_ab, f_b),   Im (M f)_a = sum_b dot2(I_ab, f_b).
+// Each bf16 x bf16 product is exact in FP32.  The thread that staged link w
+// (stage_links: one thread per link of the tile) rewrites its 9 complex
+// entries in place, 18 words in the same 72 bytes (only_dir >= 0: only the
+// links of that staged direction).
+__device__ __forceinline__ void pack_links_dot2(cx<float>* lu, int only_dir = -1) {
+  const int w = threadIdx.x;
+  if (w >= 8 * NT || (only_dir >= 0 && w / NT != only_dir)) return;
+  cx<float> u[9];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) u[e] = lu[w * 9 + e];
+  const bool dag = w >= 4 * NT;  // staged direction w / NT = 4..7: backward links
+  uint32_t* q = reinterpret_cast<uint32_t*>(lu + w * 9);
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const cx<float> m = dag ? cx<float>{u[b * 3 + a].re, -u[b * 3 + a].im} : u[a * 3 + b];
+      q[(a * 3 + b) * 2] = dwfk::bf16_pair(m.re, -m.im);
+      q[(a * 3 + b) * 2 + 1] = dwfk::bf16_pair(m.im, m.re);
+    }
+}
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float dot2(uint32_t a, uint32_t b, float c) {
+  return __builtin_amdgcn_fdot2_f32_bf16(__builtin_bit_cast(bf16x2, a),
+                                         __builtin_bit_cast(bf16x2, b), c, false);
+}
+
+// Transport of the 4 spins of one channel: f[sp * 3 + b] (activation words),
+// lw the 18 packed link words of (d, site) (unused for d = 0).  Out: for
+// colour a, re[a][h] / im[a][h] = bf16 pairs of spins (2h, 2h + 1).
+__device__ __forceinline__ void transport4(const uint32_t* lw, int d, const uint32_t* f,
+                                           uint32_t (&re)[3][2], uint32_t (&im)[3][2]) {
   if (d == 0) {
-    v[0] = f[0];
-    v[1] = f[1];
-    v[2] = f[2];
-  } else if (d <= 4) {
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-      v[a] = dwfk::row_dot(u[a * 3 + 0], u[a * 3 + 1], u[a * 3 + 2], f[0], f[1], f[2]);
-  } else {
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-      v[a] = dwfk::row_dotc(u[0 * 3 + a], u[1 * 3 + a], u[2 * 3 + a], f[0], f[1], f[2]);
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t x = f[(2 * h) * 3 + a], y = f[(2 * h + 1) * 3 + a];
+        re[a][h] = (x & 0xffffu) | (y << 16);
+        im[a][h] = (x >> 16) | (y & 0xffff0000u);
+      }
+    return;
   }
+  uint32_t L[18];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) {
+    const uint2 v = *reinterpret_cast<const uint2*>(lw + 2 * e);
+    L[2 * e] = v.x;
+    L[2 * e + 1] = v.y;
+  }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      float vr[2], vi[2];
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        const uint32_t* fs = f + (2 * h + k) * 3;
+        vr[k] = dot2(L[(a * 3 + 2) * 2], fs[2],
+                     dot2(L[(a * 3 + 1) * 2], fs[1], dot2(L[(a * 3) * 2], fs[0], 0.f)));
+        vi[k] = dot2(L[(a * 3 + 2) * 2 + 1], fs[2],
+                     dot2(L[(a * 3 + 1) * 2 + 1], fs[1], dot2(L[(a * 3) * 2 + 1], fs[0], 0.f)));
+      }
+      re[a][h] = dwfk::bf16_pair(vr[0], vr[1]);
+      im[a][h] = dwfk::bf16_pair(vi[0], vi[1]);
+    }
 }
 
 // site of the activation neighbour of tile site n along d (S index)
@@ -164,82 +224,74 @@ __device__ __forceinline__ size_t nbr_site(const int* lnb, int d, int n, int par
   return (size_t)(1 - par) * g.vh + lnb[(d - 1) * NT + n];
 }
 
-// Fill the transported field of direction d for the tile into LDS, as bf16,
-// either [col][K'] (KMAJOR = false, the B operand of the forward GEMM) or
-// [K'][col] (KMAJOR = true, the Y operand of the weight gradient).
-// k' = part*(CIN*4) + i*4 + spin, col = n*3 + colour.
-// Staging loops issue their loads CHUNK items at a time before using any
-// (a few round trips to memory per tile instead of one per item, without
-// holding a whole tile in registers next to the MFMA accumulators).
-constexpr int CHUNK = 4;
-// the largest chunk of at most `cap` items that divides `per` (items per thread)
-constexpr int chunk_for(int per, int cap) {
-  int c = cap;
-  while (c > 1 && per % c != 0) --c;
-  return c;
+// (site, channel) items of a tile spread over the work-group: PER per thread
+template <int CIN>
+struct Items {
+  static constexpr int N = NT * CIN, PER = (N + NTHR - 1) / NTHR;
+  static constexpr bool EXACT = N % NTHR == 0;
+};
+
+// Issue the loads of this thread's items of direction d: raw[j] holds the 12
+// activation words (4 spins x 3 colours) of channel i at the neighbour of
+// tile site n.  The neighbour list of the tile must be staged unless d = 0.
+template <int CIN>
+__device__ __forceinline__ void gather_items(u32x4 (&raw)[Items<CIN>::PER][3],
+                                             const uint32_t* __restrict__ in, const int* lnb,
+                                             int d, int par, int i0, const Geom& g) {
+#pragma unroll
+  for (int j = 0; j < Items<CIN>::PER; ++j) {
+    const int w = threadIdx.x + j * NTHR;
+    if (Items<CIN>::EXACT || w < Items<CIN>::N) {
+      const int n = w / CIN, i = w - n * CIN;
+      const size_t y = nbr_site(lnb, d, n, par, i0, g);
+      const u32x4* src = reinterpret_cast<const u32x4*>(in + (y * CIN + i) * 12);
+      raw[j][0] = src[0];
+      raw[j][1] = src[1];
+      raw[j][2] = src[2];
+    }
+  }
 }
 
+// Transport the gathered items of direction d and store them into LDS as
+// bf16, either [col][K'] (KMAJOR = false, the B operand of the forward GEMM)
+// or [K'][col] (KMAJOR = true, the Y operand of the weight gradient), with
+// k' = part*(CIN*4) + i*4 + spin and col = n*3 + colour.  The link of an item
+// is read from LDS once for its 4 spins, and in [col][K'] the 4 spins of a
+// colour's real (imaginary) parts are one 8-byte store.
 template <int CIN, bool KMAJOR>
-__device__ __forceinline__ void build_transported(uint16_t* sB, const uint32_t* __restrict__ in,
-                                                  const cx<float>* lu, const int* lnb, int d,
-                                                  int par, int i0, const Geom& g) {
+__device__ __forceinline__ void store_transported(uint16_t* sB,
+                                                  const u32x4 (&raw)[Items<CIN>::PER][3],
+                                                  const uint32_t* lw, int d) {
   constexpr int KP = 8 * CIN;
-  constexpr int NITEM = NT * CIN * 4, PERT = (NITEM + NTHR - 1) / NTHR;
-  constexpr int CH = chunk_for(PERT, CHUNK);
-  // no item guards when the items split evenly into whole chunks
-  constexpr bool EXACT = NITEM % NTHR == 0 && PERT % CH == 0;
-#pragma unroll 1
-  for (int j0 = 0; j0 < PERT; j0 += CH) {
-    uint32_t raw[CH][3];
 #pragma unroll
-    for (int jj = 0; jj < CH; ++jj) {
-      const int w = threadIdx.x + (j0 + jj) * NTHR;
-      if (EXACT || (j0 + jj < PERT && w < NITEM)) {
-        const int n = w / (CIN * 4), r = w - n * (CIN * 4);
-        const int i = r >> 2, sp = r & 3;
-        const size_t y = nbr_site(lnb, d, n, par, i0, g);
-        const uint32_t* src = in + (y * CIN + i) * 12 + sp * 3;
-        raw[jj][0] = src[0];
-        raw[jj][1] = src[1];
-        raw[jj][2] = src[2];
-      }
-    }
+  for (int j = 0; j < Items<CIN>::PER; ++j) {
+    const int w = threadIdx.x + j * NTHR;
+    if (!Items<CIN>::EXACT && w >= Items<CIN>::N) continue;
+    const int n = w / CIN, i = w - n * CIN;
+    const uint32_t f[12] = {raw[j][0][0], raw[j][0][1], raw[j][0][2], raw[j][0][3],
+                            raw[j][1][0], raw[j][1][1], raw[j][1][2], raw[j][1][3],
+                            raw[j][2][0], raw[j][2][1], raw[j][2][2], raw[j][2][3]};
+    uint32_t re[3][2], im[3][2];
+    transport4(lw + ((d > 0 ? d - 1 : 0) * NT + n) * 18, d, f, re, im);
 #pragma unroll
-    for (int jj = 0; jj < CH; ++jj) {
-      const int w = threadIdx.x + (j0 + jj) * NTHR;
-      if (!EXACT && (j0 + jj >= PERT || w >= NITEM)) continue;
-      const int n = w / (CIN * 4), r = w - n * (CIN * 4);
-      const int i = r >> 2, sp = r & 3;
-      cx<float> f[3], v[3];
-      f[0] = unpack_c(raw[jj][0]);
-      f[1] = unpack_c(raw[jj][1]);
-      f[2] = unpack_c(raw[jj][2]);
-      transport3(d == 0 ? nullptr : lu + ((d - 1) * NT + n) * 9, d, f, v);
-      const int kre = i * 4 + sp, kim = CIN * 4 + i * 4 + sp;
+    for (int a = 0; a < 3; ++a) {
+      const int col = n * 3 + a;
+      if (KMAJOR) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int col = n * 3 + c;
-        if (KMAJOR) {
-          sB[kre * ROWC + col] = f2bf(v[c].re);
-          sB[kim * ROWC + col] = f2bf(v[c].im);
-        } else {
-          sB[col * (KP + PADK) + kre] = f2bf(v[c].re);
-          sB[col * (KP + PADK) + kim] = f2bf(v[c].im);
+        for (int sp = 0; sp < 4; ++sp) {
+          const int sh = (sp & 1) * 16;
+          sB[(i * 4 + sp) * ROWC + col] = (uint16_t)(re[a][sp >> 1] >> sh);
+          sB[(CIN * 4 + i * 4 + sp) * ROWC + col] = (uint16_t)(im[a][sp >> 1] >> sh);
         }
+      } else {
+        *reinterpret_cast<uint2*>(sB + col * (KP + PADK) + i * 4) = uint2{re[a][0], re[a][1]};
+        *reinterpret_cast<uint2*>(sB + col * (KP + PADK) + CIN * 4 + i * 4) =
+            uint2{im[a][0], im[a][1]};
       }
     }
   }
 }
 
-__device__ __forceinline__ bf16x4 ld4(const uint16_t* p) {
-  const uint2 w = *reinterpret_cast<const uint2*>(p);
-  bf16x4 r;
-  r[0] = (short)(w.x & 0xffff);
-  r[1] = (short)(w.x >> 16);
-  r[2] = (short)(w.y & 0xffff);
-  r[3] = (short)(w.y >> 16);
-  return r;
-}
 // 8 consecutive bf16 (16-byte aligned: one LDS load)
 __device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
@@ -250,34 +302,75 @@ __device__ __forceinline__ bf16x8 ld8x2(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, uint4{a.x, a.y, b.x, b.y});
 }
 
-// LDS bytes of the forward kernel
+// Output rows of the forward layer kernel: M' = 8*COUT rows in MB = M'/16
+// blocks, the real parts (rows o*4 + spin) in blocks [0, MB/2), the imaginary
+// parts in [MB/2, MB).  When MB is a multiple of 4 ("paired"), the wave pair
+// h = 0, 1 takes real blocks [h*MB/4, (h+1)*MB/4) and the imaginary blocks of
+// the same channels, so a lane holds both halves of its output words and
+// stores them from registers; otherwise the C tile goes through LDS.
+template <int COUT>
+constexpr bool fwd_paired() { return (8 * COUT / 16) % 4 == 0; }
+
+// LDS bytes of the forward kernel: the transported tile [NCOL][K'] (+ the C
+// tile [M'][NCOL] in FP32 when not paired, in the same bytes) and the links
+__host__ __device__ constexpr size_t fwd_lds_links() { return (size_t)8 * NT * 9 * 8 + 8 * NT * 4; }
 template <int CIN, int COUT>
 constexpr size_t fwd_lds() {
-  constexpr size_t a = (size_t)(8 * COUT) * (8 * CIN + PADK) * 2;
   constexpr size_t b = (size_t)NCOL * (8 * CIN + PADK) * 2;
-  constexpr size_t c = (size_t)(8 * COUT) * NCOL * 4;
-  constexpr size_t links = (size_t)8 * NT * 9 * 8 + 8 * NT * 4;
-  return (a + b > c ? a + b : c) + links;
+  constexpr size_t c = fwd_paired<COUT>() ? 0 : (size_t)(8 * COUT) * NCOL * 4;
+  return (b > c ? b : c) + fwd_lds_links();
+}
+
+// Row block of local index i of wave pair h in the forward kernel (MB when
+// that wave has none): paired shapes take real blocks [h*MB/4, (h+1)*MB/4)
+// and the imaginary blocks of the same channels, others a contiguous range.
+template <int RM, int MB>
+__device__ __forceinline__ int fwd_block(int i, int h, bool paired) {
+  if (paired) return i < MB / 4 ? h * (MB / 4) + i : MB / 2 + h * (MB / 4) + (i - MB / 4);
+  return h * RM + i < MB ? h * RM + i : MB;
+}
+
+// The weight fragments of direction d for this wave's row blocks
+template <int RM, int K32, int MB>
+__device__ __forceinline__ void load_frags(u32x4 (&af)[RM][K32], const u32x4* __restrict__ A,
+                                           int d, int h, int lane, bool paired) {
+#pragma unroll
+  for (int i = 0; i < RM; ++i) {
+    const int blk = fwd_block<RM, MB>(i, h, paired);
+    if (blk < MB)
+#pragma unroll
+      for (int kb = 0; kb < K32; ++kb) af[i][kb] = A[((d * MB + blk) * K32 + kb) * 64 + lane];
+  }
 }
 
 // One layer on one tile of NT sites per work-group:
-// out[S][o][spin][colour] = sum_d W_d T_d in.  A: [9][M'][K'] bf16.
+// out[S][o][spin][colour] = sum_d W_d T_d in.
+// A: the real forms [9][M'][K'] in MFMA fragment order, one 16-byte word per
+// (d, row block, K step of 32, lane): A[d][16*blk + l%16][32*kb + 8*(l/16) ..
+// + 7] at ((d*MB + blk)*K32 + kb)*64 + l, so that each wave loads its
+// fragments straight from memory (no LDS stage; every work-group reads the
+// same few KB of weights, which stay in L2).
 // grid.x = 2 * vh / NT tiles (first all even-parity tiles, then odd).
+// A direction's weight fragments in registers (64 VGPRs at K' = M' = 128)
+// hold the kernel to two waves per SIMD; so does the LDS of the unpaired
+// shapes.  Pipelining the transport of the next direction into a second B
+// tile, or loading the fragments one K step ahead to fit three work-groups
+// per CU, timed the same or slower.
 template <int CIN, int COUT>
-__global__ void __launch_bounds__(NTHR)
+__global__ void __launch_bounds__(NTHR, 2)
 k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
-          const uint16_t* __restrict__ A, const typename dwfk::P32::S* __restrict__ U, Geom g) {
+          const u32x4* __restrict__ A, const typename dwfk::P32::S* __restrict__ U, Geom g) {
   constexpr int MP = 8 * COUT, KP = 8 * CIN;
   constexpr int MB = MP / 16, NB = NCOL / 16;
-  constexpr int K32 = KP / 32, K16 = (KP % 32) / 16;  // K steps of 32, a tail of 16
-  static_assert(KP % 16 == 0 && NB % 2 == 0 && ((KP + PADK) * 2) % 16 == 0, "tile shapes");
-  // this wave's output blocks: rows [mb0, mb0 + RM) x columns [nb0, nb0 + RN)
-  constexpr int RM = (MB + 1) / 2, RN = NB / 2;
+  constexpr int K32 = KP / 32;
+  constexpr bool PAIRED = fwd_paired<COUT>();
+  static_assert(KP % 32 == 0 && NB % 2 == 0 && ((KP + PADK) * 2) % 16 == 0, "tile shapes");
+  // this wave's output blocks: RM row blocks x columns [nb0, nb0 + RN)
+  constexpr int RM = PAIRED ? MB / 2 : (MB + 1) / 2, RH = MB / 4, RN = NB / 2;
   extern __shared__ __align__(16) unsigned char smem[];
-  uint16_t* sA = reinterpret_cast<uint16_t*>(smem);
-  uint16_t* sB = sA + MP * (KP + PADK);
-  float* sC = reinterpret_cast<float*>(smem);  // after the last GEMM
-  constexpr size_t big = fwd_lds<CIN, COUT>() - ((size_t)8 * NT * 9 * 8 + 8 * NT * 4);
+  uint16_t* sB = reinterpret_cast<uint16_t*>(smem);
+  float* sC = reinterpret_cast<float*>(smem);  // after the last GEMM (not paired)
+  constexpr size_t big = fwd_lds<CIN, COUT>() - fwd_lds_links();
   cx<float>* lu = reinterpret_cast<cx<float>*>(smem + big);
   int* lnb = reinterpret_cast<int*>(smem + big + (size_t)8 * NT * 9 * 8);
 
@@ -285,44 +378,31 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
   const int par = blockIdx.x / tiles_per_par;
   const int i0 = (blockIdx.x - par * tiles_per_par) * NT;
   dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par);
+  pack_links_dot2(lu);
+  const uint32_t* lw = reinterpret_cast<const uint32_t*>(lu);
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int mb0 = (wave & 1) * RM, nb0 = (wave >> 1) * RN;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = wave & 1;
+  const int nb0 = (wave >> 1) * RN;
+  auto blk_of = [&](int i) { return fwd_block<RM, MB>(i, h, PAIRED); };
   f32x4 acc[RM][RN];
 #pragma unroll
   for (int i = 0; i < RM; ++i)
 #pragma unroll
     for (int j = 0; j < RN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
+  // Per direction: the weight fragments are loaded first (their latency is
+  // covered by the transport), the next direction's neighbour activations
+  // after the B tile is built (covered by the products).
+  u32x4 raw[Items<CIN>::PER][3];
+  gather_items<CIN>(raw, in, lnb, 0, par, i0, g);  // d = 0: own sites
   for (int d = 0; d < NDIR; ++d) {
-    __syncthreads();  // links staged / previous direction's operands consumed
-    {
-      // the direction's weights: every load issued before the stores
-      constexpr int NA = MP * KP / 8, PA = (NA + NTHR - 1) / NTHR;
-      constexpr int CH = chunk_for(PA, CHUNK);
-      constexpr bool EXACT = NA % NTHR == 0 && PA % CH == 0;
-      const uint4* src = reinterpret_cast<const uint4*>(A + (size_t)d * MP * KP);
-#pragma unroll 1
-      for (int j0 = 0; j0 < PA; j0 += CH) {
-        uint4 wv[CH];
-#pragma unroll
-        for (int jj = 0; jj < CH; ++jj) {
-          const int w = threadIdx.x + (j0 + jj) * NTHR;
-          if (EXACT || (j0 + jj < PA && w < NA)) wv[jj] = src[w];
-        }
-#pragma unroll
-        for (int jj = 0; jj < CH; ++jj) {
-          const int w = threadIdx.x + (j0 + jj) * NTHR;
-          if (EXACT || (j0 + jj < PA && w < NA)) {
-            const int m = w / (KP / 8), k8 = w - m * (KP / 8);
-            *reinterpret_cast<uint4*>(sA + m * (KP + PADK) + k8 * 8) = wv[jj];
-          }
-        }
-      }
-    }
-    build_transported<CIN, false>(sB, in, lu, lnb, d, par, i0, g);
+    u32x4 af[RM][K32];
+    load_frags<RM, K32, MB>(af, A, d, h, lane, PAIRED);
+    __syncthreads();  // links staged / previous direction's B tile consumed
+    store_transported<CIN, false>(sB, raw, lw, d);
     __syncthreads();
-    const uint16_t* pa = sA + (mb0 * 16 + (lane & 15)) * (KP + PADK);
+    if (d + 1 < NDIR) gather_items<CIN>(raw, in, lnb, d + 1, par, i0, g);
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the products
     const uint16_t* pb = sB + (nb0 * 16 + (lane & 15)) * (KP + PADK);
 #pragma unroll
     for (int kb = 0; kb < K32; ++kb) {
@@ -332,50 +412,52 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
       for (int j = 0; j < RN; ++j) bfr[j] = ld8(pb + j * 16 * (KP + PADK) + k0);
 #pragma unroll
       for (int i = 0; i < RM; ++i) {
-        if (mb0 + i < MB) {
-          const bf16x8 af = ld8(pa + i * 16 * (KP + PADK) + k0);
+        if (blk_of(i) < MB) {
+          const bf16x8 a8 = __builtin_bit_cast(bf16x8, af[i][kb]);
 #pragma unroll
           for (int j = 0; j < RN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfr[j], acc[i][j], 0, 0, 0);
-        }
-      }
-    }
-    if constexpr (K16 > 0) {
-      const int k0 = K32 * 32 + 4 * (lane >> 4);
-      bf16x4 bfr[RN];
-#pragma unroll
-      for (int j = 0; j < RN; ++j) bfr[j] = ld4(pb + j * 16 * (KP + PADK) + k0);
-#pragma unroll
-      for (int i = 0; i < RM; ++i) {
-        if (mb0 + i < MB) {
-          const bf16x4 af = ld4(pa + i * 16 * (KP + PADK) + k0);
-#pragma unroll
-          for (int j = 0; j < RN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(af, bfr[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a8, bfr[j], acc[i][j], 0, 0, 0);
         }
       }
     }
   }
-  __syncthreads();
-  // C (lane: rows 4(l/16)..+3 of column l%16) -> LDS [M'][NCOL]
+  // C: lane l holds rows 4(l/16)..+3 of column l%16 of each block
+  if constexpr (PAIRED) {
+    // out[S][o][spin][colour] = (re row o*4+spin, im row COUT*4 + o*4+spin)
 #pragma unroll
-  for (int i = 0; i < RM; ++i) {
-    if (mb0 + i < MB) {
+    for (int i = 0; i < RH; ++i) {
+      const int o = (h * RH + i) * 4 + (lane >> 4);
 #pragma unroll
-      for (int j = 0; j < RN; ++j)
+      for (int j = 0; j < RN; ++j) {
+        const int col = (nb0 + j) * 16 + (lane & 15), n = col / 3, c = col - n * 3;
+        uint32_t* dst = out + (((size_t)par * g.vh + i0 + n) * COUT + o) * 12 + c;
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          sC[((mb0 + i) * 16 + 4 * (lane >> 4) + r) * NCOL + (nb0 + j) * 16 + (lane & 15)] =
-              acc[i][j][r];
+          dst[r * 3] = pack_c(cx<float>{acc[i][j][r], acc[i + RH][j][r]});
+      }
     }
-  }
-  __syncthreads();
-  // out[S][o][spin][colour]: re row o*4+spin, im row COUT*4 + o*4 + spin
-  for (int w = threadIdx.x; w < NT * COUT * 12; w += NTHR) {
-    const int n = w / (COUT * 12), r = w - n * (COUT * 12);
-    const int os = r / 3, c = r - os * 3;
-    const float re = sC[os * NCOL + n * 3 + c], im = sC[(COUT * 4 + os) * NCOL + n * 3 + c];
-    out[((size_t)par * g.vh + i0 + n) * (COUT * 12) + r] = pack_c(cx<float>{re, im});
+  } else {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < RM; ++i) {
+      const int blk = blk_of(i);
+      if (blk < MB) {
+#pragma unroll
+        for (int j = 0; j < RN; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            sC[(blk * 16 + 4 * (lane >> 4) + r) * NCOL + (nb0 + j) * 16 + (lane & 15)] =
+                acc[i][j][r];
+      }
+    }
+    __syncthreads();
+    // out[S][o][spin][colour]: re row o*4+spin, im row COUT*4 + o*4 + spin
+    for (int w = threadIdx.x; w < NT * COUT * 12; w += NTHR) {
+      const int n = w / (COUT * 12), r = w - n * (COUT * 12);
+      const int os = r / 3, c = r - os * 3;
+      const float re = sC[os * NCOL + n * 3 + c], im = sC[(COUT * 4 + os) * NCOL + n * 3 + c];
+      out[((size_t)par * g.vh + i0 + n) * (COUT * 12) + r] = pack_c(cx<float>{re, im});
+    }
   }
 }
 
@@ -386,11 +468,30 @@ constexpr size_t gw_lds() {
   return x + y + (size_t)8 * NT * 9 * 8 + 8 * NT * 4;
 }
 
+// The output gradient words of one tile (contiguous in gout: NT sites x
+// COUT channels x 12), 16 bytes at a time: PER per thread
+template <int COUT>
+struct XTile {
+  static constexpr int N4 = NT * COUT * 12 / 4, PER = (N4 + NTHR - 1) / NTHR;
+  static constexpr bool EXACT = N4 % NTHR == 0;
+};
+template <int COUT>
+__device__ __forceinline__ void load_x(u32x4 (&xv)[XTile<COUT>::PER],
+                                       const uint32_t* __restrict__ gout, int par, int i0,
+                                       const Geom& g) {
+  const u32x4* src = reinterpret_cast<const u32x4*>(gout + ((size_t)par * g.vh + i0) * (COUT * 12));
+#pragma unroll
+  for (int j = 0; j < XTile<COUT>::PER; ++j) {
+    const int w4 = threadIdx.x + j * NTHR;
+    if (XTile<COUT>::EXACT || w4 < XTile<COUT>::N4) xv[j] = src[w4];
+  }
+}
+
 // Weight gradient partials: for direction d = blockIdx.y and the tiles
 // t = blockIdx.x, blockIdx.x + gridDim.x, ...:
 // part[d][blockIdx.x][M'][K'] = sum X[M'][cols] Y_d[K'][cols]^T
 template <int CIN, int COUT>
-__global__ void __launch_bounds__(NTHR)
+__global__ void __launch_bounds__(NTHR, 2)
 k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
          const uint32_t* __restrict__ in, const typename dwfk::P32::S* __restrict__ U, Geom g) {
   constexpr int MP = 8 * COUT, KP = 8 * CIN;
@@ -412,42 +513,47 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
   for (int i = 0; i < RM; ++i)
 #pragma unroll
     for (int j = 0; j < RK; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // X of the next tile is loaded into registers before this tile's products
+  u32x4 xv[XTile<COUT>::PER];
+  if (blockIdx.x < ntiles)
+    load_x<COUT>(xv, gout, blockIdx.x / tiles_per_par,
+                 (blockIdx.x - blockIdx.x / tiles_per_par * tiles_per_par) * NT, g);
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int par = t / tiles_per_par;
     const int i0 = (t - par * tiles_per_par) * NT;
     __syncthreads();  // previous tile's operands consumed
-    dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par);
-    // X: the output gradient of the tile, [M'][col]; the tile is contiguous
-    // in gout, and every load is issued before the stores
-    {
-      constexpr int NX = NT * COUT * 12, PX = (NX + NTHR - 1) / NTHR;
-      constexpr int CH = chunk_for(PX, 2 * CHUNK);
-      constexpr bool EXACT = NX % NTHR == 0 && PX % CH == 0;
-      const uint32_t* src = gout + ((size_t)par * g.vh + i0) * (COUT * 12);
-#pragma unroll 1
-      for (int j0 = 0; j0 < PX; j0 += CH) {
-        uint32_t xv[CH];
+    // only direction d's links and neighbours are read (none for d = 0)
+    if (d > 0) {
+      dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par, d - 1);
+      pack_links_dot2(lu, d - 1);
+    }
+    // X: the output gradient of the tile, [M'][col]
 #pragma unroll
-        for (int jj = 0; jj < CH; ++jj) {
-          const int w = threadIdx.x + (j0 + jj) * NTHR;
-          if (EXACT || (j0 + jj < PX && w < NX)) xv[jj] = src[w];
-        }
+    for (int j = 0; j < XTile<COUT>::PER; ++j) {
+      const int w4 = threadIdx.x + j * NTHR;
+      if (!XTile<COUT>::EXACT && w4 >= XTile<COUT>::N4) continue;
 #pragma unroll
-        for (int jj = 0; jj < CH; ++jj) {
-          const int w = threadIdx.x + (j0 + jj) * NTHR;
-          if (EXACT || (j0 + jj < PX && w < NX)) {
-            const int n = w / (COUT * 12), r = w - n * (COUT * 12);
-            const int os = r / 3, c = r - os * 3;
-            const cx<float> v = unpack_c(xv[jj]);
-            sX[os * ROWC + n * 3 + c] = f2bf(v.re);
-            sX[(COUT * 4 + os) * ROWC + n * 3 + c] = f2bf(v.im);
-          }
-        }
+      for (int e = 0; e < 4; ++e) {
+        const int w = 4 * w4 + e;
+        const int n = w / (COUT * 12), r = w - n * (COUT * 12);
+        const int os = r / 3, c = r - os * 3;
+        const uint32_t x = xv[j][e];
+        sX[os * ROWC + n * 3 + c] = (uint16_t)(x & 0xffffu);
+        sX[(COUT * 4 + os) * ROWC + n * 3 + c] = (uint16_t)(x >> 16);
       }
     }
     __syncthreads();  // links staged
-    build_transported<CIN, true>(sY, in, lu, lnb, d, par, i0, g);
+    {
+      u32x4 raw[Items<CIN>::PER][3];
+      gather_items<CIN>(raw, in, lnb, d, par, i0, g);
+      store_transported<CIN, true>(sY, raw, reinterpret_cast<const uint32_t*>(lu), d);
+    }
     __syncthreads();
+    {
+      const int tn = t + gridDim.x;
+      if (tn < ntiles) load_x<COUT>(xv, gout, tn / tiles_per_par, (tn - tn / tiles_per_par * tiles_per_par) * NT, g);
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the products
     const uint16_t* px = sX + (mb0 * 16 + (lane & 15)) * ROWC;
     const uint16_t* py = sY + (kb0 * 16 + (lane & 15)) * ROWC;
 #pragma unroll

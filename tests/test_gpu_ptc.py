@@ -3,8 +3,9 @@ restatement (oracle/ptc_net.py), its training, and the flexible CG it
 preconditions.
 
 Bars (the oracle runs with the same bf16-rounded weights, so the remaining
-difference is the bf16 rounding of activations and transported fields, about
-2^-9 = 2e-3 per rounding, a handful of roundings per layer):
+difference is the bf16 rounding of activations, of the links in the
+transport and of the transported fields, about 2^-9 = 2e-3 per rounding, a
+handful of roundings per layer):
   * network output vs oracle, normwise: < 3e-2
   * loss vs oracle: < 3e-2 relative; weight gradient per layer, normwise:
     < 6e-2
