@@ -72,9 +72,13 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b))
 
 
+# Ls 4: layers <4,16>, <16,16>, <16,4> (one K step; 4 output channels take
+# the LDS output path); Ls 12: <12,16>, <16,16>, <16,12> (three K steps, 1.5
+# items per thread, 12 output channels through LDS, 16 from registers)
+@pytest.mark.parametrize("ls", [4, 12])
 @pytest.mark.parametrize("action", ["dwf", "mobius"])
-def test_network_forward_matches_oracle(action):
-    dims, ls = (4, 4, 4, 8), 4
+def test_network_forward_matches_oracle(action, ls):
+    dims = (4, 4, 4, 8)
     say(f"[gpu] network forward {action} {dims} ls={ls}")
     U, g = make(action, dims, ls)
     g.ptc_init(1, 0.02)
@@ -88,10 +92,11 @@ def test_network_forward_matches_oracle(action):
     assert err < 3e-2
 
 
+@pytest.mark.parametrize("ls", [4, 12])
 @pytest.mark.parametrize("action", ["dwf", "mobius"])
-def test_network_gradients_match_oracle(action):
-    dims, ls = (4, 4, 4, 8), 4
-    say(f"[gpu] network gradients {action}")
+def test_network_gradients_match_oracle(action, ls):
+    dims = (4, 4, 4, 8)
+    say(f"[gpu] network gradients {action} ls={ls}")
     U, g = make(action, dims, ls)
     g.ptc_init(1, 0.02)
     rng = np.random.default_rng(4)
