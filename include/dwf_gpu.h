@@ -281,8 +281,9 @@ int dwfgpu_invcg_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int sloppy, doubl
  * directions: each step z = N(r), w = A z made orthogonal to the cycle's
  * earlier w_j (complex Gram-Schmidt; the network is not Hermitian), x and r
  * updated to minimise |r| over the cycle's space; FP64 vectors and operator.
- * After 10 steps that cut |r|^2 by less than 0.1 %, or without a network,
- * plain CG (dwfgpu_invcg) finishes from the current solution; *iters counts
+ * After 10 steps that cut |r|^2 by less than 0.1 %, after a cycle that does
+ * not halve |r|^2, or without a network, plain CG (dwfgpu_invcg) finishes
+ * from the current solution; *iters counts
  * both (CPS's convention, steps + 1), *net_steps the steps the network
  * helped, *restarts the completed cycles.  No CPS counterpart.            */
 int dwfgpu_invgcr_ptc_dev(dwfgpu_t h, const dwfgpu_cg_arg *arg, int nkrylov, dwfgpu_vec_t sol,
