@@ -28,7 +28,10 @@
 // K step and issues every product between them, so each fragment is read
 // once per step, not once per output block.  The forward
 // layer reads its weight fragments straight from memory (pre-arranged in
-// fragment order) and only the transported field goes through LDS.
+// fragment order) and only the transported field goes through LDS.  The
+// weight gradient keeps both operands as [col][row] tiles (written with 8-byte
+// stores, like the forward's B tile) and reads its fragments with gfx950's
+// transposing LDS loads (ds_read_b64_tr_b16).
 // Transport: the links are staged as packed bf16 pairs so that each complex
 // multiply-add of U phi is two v_dot2_f32_bf16 on the activation words as
 // stored (FP32 accumulation).
@@ -45,10 +48,6 @@ constexpr int NT = 32;          // sites per tile
 constexpr int NCOL = 3 * NT;    // GEMM columns per tile
 constexpr int NTHR = 256;       // threads (4 waves) per work-group
 constexpr int PADK = 8;         // LDS row padding of [col][K'] tiles (bf16 elements)
-// row stride of the [K'][col] / [M'][col] tiles of the weight gradient: 100
-// bf16 = 50 dwords, so the consecutive rows a wavefront writes while building
-// a tile land in different LDS banks (a stride of 104 repeats every 16 rows)
-constexpr int ROWC = NCOL + 4;
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -254,12 +253,12 @@ __device__ __forceinline__ void gather_items(u32x4 (&raw)[Items<CIN>::PER][3],
 }
 
 // Transport the gathered items of direction d and store them into LDS as
-// bf16, either [col][K'] (KMAJOR = false, the B operand of the forward GEMM)
-// or [K'][col] (KMAJOR = true, the Y operand of the weight gradient), with
-// k' = part*(CIN*4) + i*4 + spin and col = n*3 + colour.  The link of an item
-// is read from LDS once for its 4 spins, and in [col][K'] the 4 spins of a
+// bf16 in [col][K'] (row stride K' + PADK; the B operand of the forward GEMM
+// and, read through the transposing loads, the Y operand of the weight
+// gradient), with k' = part*(CIN*4) + i*4 + spin and col = n*3 + colour.  The
+// link of an item is read from LDS once for its 4 spins, and the 4 spins of a
 // colour's real (imaginary) parts are one 8-byte store.
-template <int CIN, bool KMAJOR>
+template <int CIN>
 __device__ __forceinline__ void store_transported(uint16_t* sB,
                                                   const u32x4 (&raw)[Items<CIN>::PER][3],
                                                   const uint32_t* lw, int d) {
@@ -277,18 +276,9 @@ __device__ __forceinline__ void store_transported(uint16_t* sB,
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int col = n * 3 + a;
-      if (KMAJOR) {
-#pragma unroll
-        for (int sp = 0; sp < 4; ++sp) {
-          const int sh = (sp & 1) * 16;
-          sB[(i * 4 + sp) * ROWC + col] = (uint16_t)(re[a][sp >> 1] >> sh);
-          sB[(CIN * 4 + i * 4 + sp) * ROWC + col] = (uint16_t)(im[a][sp >> 1] >> sh);
-        }
-      } else {
-        *reinterpret_cast<uint2*>(sB + col * (KP + PADK) + i * 4) = uint2{re[a][0], re[a][1]};
-        *reinterpret_cast<uint2*>(sB + col * (KP + PADK) + CIN * 4 + i * 4) =
-            uint2{im[a][0], im[a][1]};
-      }
+      *reinterpret_cast<uint2*>(sB + col * (KP + PADK) + i * 4) = uint2{re[a][0], re[a][1]};
+      *reinterpret_cast<uint2*>(sB + col * (KP + PADK) + CIN * 4 + i * 4) =
+          uint2{im[a][0], im[a][1]};
     }
   }
 }
@@ -297,12 +287,6 @@ __device__ __forceinline__ void store_transported(uint16_t* sB,
 __device__ __forceinline__ bf16x8 ld8(const uint16_t* p) {
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
 }
-// 8 consecutive bf16 at an 8-byte boundary (two LDS loads)
-__device__ __forceinline__ bf16x8 ld8x2(const uint16_t* p) {
-  const uint2 a = *reinterpret_cast<const uint2*>(p), b = *reinterpret_cast<const uint2*>(p + 4);
-  return __builtin_bit_cast(bf16x8, uint4{a.x, a.y, b.x, b.y});
-}
-
 // Output rows of the forward layer kernel: M' = 8*COUT rows in MB = M'/16
 // blocks, the real parts (rows o*4 + spin) in blocks [0, MB/2), the imaginary
 // parts in [MB/2, MB).  When MB is a multiple of 4 ("paired"), the wave pair
@@ -411,7 +395,7 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
     u32x4 af[RM][K32];
     load_frags<RM, K32, MB>(af, A, d, h, lane, PAIRED);
     __syncthreads();  // links staged / previous direction's B tile consumed
-    store_transported<CIN, false>(sB, raw, lw, d);
+    store_transported<CIN>(sB, raw, lw, d);
     __syncthreads();
     if (d + 1 < NDIR) gather_items<CIN>(raw, in, lnb, d + 1, par, i0, g);
     __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the products
@@ -475,27 +459,49 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
 
 template <int CIN, int COUT>
 constexpr size_t gw_lds() {
-  constexpr size_t x = (size_t)(8 * COUT) * ROWC * 2;
-  constexpr size_t y = (size_t)(8 * CIN) * ROWC * 2;
+  constexpr size_t x = (size_t)NCOL * (8 * COUT + PADK) * 2;
+  constexpr size_t y = (size_t)NCOL * (8 * CIN + PADK) * 2;
   return x + y + (size_t)8 * NT * 9 * 8 + 8 * NT * 4;
 }
 
-// The output gradient words of one tile (contiguous in gout: NT sites x
-// COUT channels x 12), 16 bytes at a time: PER per thread
+// One MFMA operand fragment (8 bf16: rows r0 + l%16, columns c0 + 8(l/16) ..
+// + 7 of the contraction) from a [col][row] LDS tile with row stride RS, by
+// two transposing reads (ds_read_b64_tr_b16).  Within each 16-lane group,
+// lane j receives in slot e element j%4 of the 8-byte chunk addressed by lane
+// 4e + j/4, so lane i addresses column c0 + 8(l/16) + 4h + i/4 at rows
+// r0 + 4(i%4) .. + 3 for the half h of the 8 columns.
+typedef short v4i16 __attribute__((ext_vector_type(4)));
+template <int RS>
+__device__ __forceinline__ bf16x8 ld_frag_tr(const uint16_t* t, int r0, int c0, int lane) {
+  const int i = lane & 15, c = c0 + 8 * (lane >> 4) + (i >> 2), r = r0 + 4 * (i & 3);
+  typedef __attribute__((address_space(3))) v4i16 lds_v4i16;
+  const v4i16 a = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4i16*)(t + c * RS + r));
+  const v4i16 b = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4i16*)(t + (c + 4) * RS + r));
+  typedef short v8i16 __attribute__((ext_vector_type(8)));
+  return __builtin_bit_cast(bf16x8, v8i16{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]});
+}
+
+// The output gradient of one tile (contiguous in gout: NT sites x COUT
+// channels x 12 words), one (site, channel) item of 12 words (three 16-byte
+// loads) at a time: PER per thread
 template <int COUT>
 struct XTile {
-  static constexpr int N4 = NT * COUT * 12 / 4, PER = (N4 + NTHR - 1) / NTHR;
-  static constexpr bool EXACT = N4 % NTHR == 0;
+  static constexpr int N = NT * COUT, PER = (N + NTHR - 1) / NTHR;
+  static constexpr bool EXACT = N % NTHR == 0;
 };
 template <int COUT>
-__device__ __forceinline__ void load_x(u32x4 (&xv)[XTile<COUT>::PER],
+__device__ __forceinline__ void load_x(u32x4 (&xv)[XTile<COUT>::PER][3],
                                        const uint32_t* __restrict__ gout, int par, int i0,
                                        const Geom& g) {
   const u32x4* src = reinterpret_cast<const u32x4*>(gout + ((size_t)par * g.vh + i0) * (COUT * 12));
 #pragma unroll
   for (int j = 0; j < XTile<COUT>::PER; ++j) {
-    const int w4 = threadIdx.x + j * NTHR;
-    if (XTile<COUT>::EXACT || w4 < XTile<COUT>::N4) xv[j] = src[w4];
+    const int w = threadIdx.x + j * NTHR;
+    if (XTile<COUT>::EXACT || w < XTile<COUT>::N) {
+      xv[j][0] = src[3 * w];
+      xv[j][1] = src[3 * w + 1];
+      xv[j][2] = src[3 * w + 2];
+    }
   }
 }
 
@@ -508,13 +514,14 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
          const uint32_t* __restrict__ in, const typename dwfk::P32::S* __restrict__ U, Geom g) {
   constexpr int MP = 8 * COUT, KP = 8 * CIN;
   constexpr int MB = MP / 16, KB2 = KP / 16, CB = NCOL / 32;  // contraction over columns
-  static_assert(NCOL % 32 == 0, "column steps of 32");
+  constexpr int XS = MP + PADK, YS = KP + PADK;  // row strides of the [col][row] tiles
+  static_assert(NCOL % 32 == 0 && (XS * 2) % 8 == 0 && (YS * 2) % 8 == 0, "tile shapes");
   // this wave's output blocks: rows [mb0, mb0 + RM) x columns [kb0, kb0 + RK)
   constexpr int RM = (MB + 1) / 2, RK = (KB2 + 1) / 2;
   extern __shared__ __align__(16) unsigned char smem[];
-  uint16_t* sX = reinterpret_cast<uint16_t*>(smem);
-  uint16_t* sY = sX + MP * ROWC;
-  cx<float>* lu = reinterpret_cast<cx<float>*>(smem + (size_t)(MP + KP) * ROWC * 2);
+  uint16_t* sX = reinterpret_cast<uint16_t*>(smem);  // X^T: [col][M']
+  uint16_t* sY = sX + NCOL * XS;                     // Y^T: [col][K']
+  cx<float>* lu = reinterpret_cast<cx<float>*>(smem + (size_t)NCOL * (XS + YS) * 2);
   int* lnb = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(lu) + (size_t)8 * NT * 9 * 8);
   const int d = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -526,7 +533,7 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
 #pragma unroll
     for (int j = 0; j < RK; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   // X of the next tile is loaded into registers before this tile's products
-  u32x4 xv[XTile<COUT>::PER];
+  u32x4 xv[XTile<COUT>::PER][3];
   if (blockIdx.x < ntiles)
     load_x<COUT>(xv, gout, blockIdx.x / tiles_per_par,
                  (blockIdx.x - blockIdx.x / tiles_per_par * tiles_per_par) * NT, g);
@@ -539,26 +546,31 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
       dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par, d - 1);
       pack_links_dot2(lu, d - 1);
     }
-    // X: the output gradient of the tile, [M'][col]
+    // X^T of the tile: item (site n, channel o) holds words spin*3 + colour;
+    // the 4 spins of a colour are 4 consecutive rows o*4 + spin (real parts)
+    // and COUT*4 + o*4 + spin (imaginary parts): two 8-byte stores per colour
 #pragma unroll
     for (int j = 0; j < XTile<COUT>::PER; ++j) {
-      const int w4 = threadIdx.x + j * NTHR;
-      if (!XTile<COUT>::EXACT && w4 >= XTile<COUT>::N4) continue;
+      const int w = threadIdx.x + j * NTHR;
+      if (!XTile<COUT>::EXACT && w >= XTile<COUT>::N) continue;
+      const int n = w / COUT, o = w - n * COUT;
+      const uint32_t f[12] = {xv[j][0][0], xv[j][0][1], xv[j][0][2], xv[j][0][3],
+                              xv[j][1][0], xv[j][1][1], xv[j][1][2], xv[j][1][3],
+                              xv[j][2][0], xv[j][2][1], xv[j][2][2], xv[j][2][3]};
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int w = 4 * w4 + e;
-        const int n = w / (COUT * 12), r = w - n * (COUT * 12);
-        const int os = r / 3, c = r - os * 3;
-        const uint32_t x = xv[j][e];
-        sX[os * ROWC + n * 3 + c] = (uint16_t)(x & 0xffffu);
-        sX[(COUT * 4 + os) * ROWC + n * 3 + c] = (uint16_t)(x >> 16);
+      for (int c = 0; c < 3; ++c) {
+        uint16_t* row = sX + (n * 3 + c) * XS;
+        *reinterpret_cast<uint2*>(row + o * 4) =
+            uint2{(f[c] & 0xffffu) | (f[3 + c] << 16), (f[6 + c] & 0xffffu) | (f[9 + c] << 16)};
+        *reinterpret_cast<uint2*>(row + COUT * 4 + o * 4) =
+            uint2{(f[c] >> 16) | (f[3 + c] & 0xffff0000u), (f[6 + c] >> 16) | (f[9 + c] & 0xffff0000u)};
       }
     }
     __syncthreads();  // links staged
     {
       u32x4 raw[Items<CIN>::PER][3];
       gather_items<CIN>(raw, in, lnb, d, par, i0, g);
-      store_transported<CIN, true>(sY, raw, reinterpret_cast<const uint32_t*>(lu), d);
+      store_transported<CIN>(sY, raw, reinterpret_cast<const uint32_t*>(lu), d);
     }
     __syncthreads();
     {
@@ -566,20 +578,16 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
       if (tn < ntiles) load_x<COUT>(xv, gout, tn / tiles_per_par, (tn - tn / tiles_per_par * tiles_per_par) * NT, g);
     }
     __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the products
-    const uint16_t* px = sX + (mb0 * 16 + (lane & 15)) * ROWC;
-    const uint16_t* py = sY + (kb0 * 16 + (lane & 15)) * ROWC;
 #pragma unroll
     for (int cb = 0; cb < CB; ++cb) {
-      // rows of ROWC = 100 bf16 are 8-byte aligned: two loads per fragment
-      const int c0 = cb * 32 + 8 * (lane >> 4);
       bf16x8 yf[RK];
 #pragma unroll
       for (int j = 0; j < RK; ++j)
-        if (kb0 + j < KB2) yf[j] = ld8x2(py + j * 16 * ROWC + c0);
+        if (kb0 + j < KB2) yf[j] = ld_frag_tr<YS>(sY, (kb0 + j) * 16, cb * 32, lane);
 #pragma unroll
       for (int i = 0; i < RM; ++i) {
         if (mb0 + i < MB) {
-          const bf16x8 xf = ld8x2(px + i * 16 * ROWC + c0);
+          const bf16x8 xf = ld_frag_tr<XS>(sX, (mb0 + i) * 16, cb * 32, lane);
 #pragma unroll
           for (int j = 0; j < RK; ++j)
             if (kb0 + j < KB2)
