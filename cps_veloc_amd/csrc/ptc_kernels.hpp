@@ -253,12 +253,12 @@ __device__ __forceinline__ void gather_items(u32x4 (&raw)[Items<CIN>::PER][3],
 }
 
 // Transport the gathered items of direction d and store them into LDS as
-// bf16 in [col][K'] (row stride K' + PADK; the B operand of the forward GEMM
+// bf16 in [col][K'] (row stride RS; the B operand of the forward GEMM
 // and, read through the transposing loads, the Y operand of the weight
 // gradient), with k' = part*(CIN*4) + i*4 + spin and col = n*3 + colour.  The
 // link of an item is read from LDS once for its 4 spins, and the 4 spins of a
 // colour's real (imaginary) parts are one 8-byte store.
-template <int CIN>
+template <int CIN, int RS = 8 * CIN + PADK>
 __device__ __forceinline__ void store_transported(uint16_t* sB,
                                                   const u32x4 (&raw)[Items<CIN>::PER][3],
                                                   const uint32_t* lw, int d) {
@@ -276,9 +276,8 @@ __device__ __forceinline__ void store_transported(uint16_t* sB,
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int col = n * 3 + a;
-      *reinterpret_cast<uint2*>(sB + col * (KP + PADK) + i * 4) = uint2{re[a][0], re[a][1]};
-      *reinterpret_cast<uint2*>(sB + col * (KP + PADK) + CIN * 4 + i * 4) =
-          uint2{im[a][0], im[a][1]};
+      *reinterpret_cast<uint2*>(sB + col * RS + i * 4) = uint2{re[a][0], re[a][1]};
+      *reinterpret_cast<uint2*>(sB + col * RS + CIN * 4 + i * 4) = uint2{im[a][0], im[a][1]};
     }
   }
 }
@@ -343,7 +342,7 @@ __device__ __forceinline__ void load_frags(u32x4 (&af)[RM][K32], const u32x4* __
 // (d, row block, K step of 32, lane): A[d][16*blk + l%16][32*kb + 8*(l/16) ..
 // + 7] at ((d*MB + blk)*K32 + kb)*64 + l, so that each wave loads its
 // fragments straight from memory (no LDS stage; every work-group reads the
-// same few KB of weights, which stay in L2).
+// same weights, 32 KB per direction at K' = M' = 128, which stay in L2).
 // grid.x = 2 * vh / NT tiles (first all even-parity tiles, then odd).
 // On the 2 x 2 grid a direction's weight fragments in registers (64 VGPRs
 // at K' = M' = 128) hold the kernel to two waves per SIMD, as does the LDS
@@ -570,7 +569,7 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
     {
       u32x4 raw[Items<CIN>::PER][3];
       gather_items<CIN>(raw, in, lnb, d, par, i0, g);
-      store_transported<CIN>(sY, raw, reinterpret_cast<const uint32_t*>(lu), d);
+      store_transported<CIN, YS>(sY, raw, reinterpret_cast<const uint32_t*>(lu), d);
     }
     __syncthreads();
     {
