@@ -717,7 +717,7 @@ __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
                                             int* __restrict__ lnb,
                                             const typename P::S* __restrict__ U, const Geom& g,
                                             int blk, int ns, int nsets, int p_first,
-                                            int only_dir = -1) {
+                                            int only_dir = -1, int dst_base = 0) {
   typedef typename P::R R;
   constexpr int NE = P::kLinkElems;
   const int tid = threadIdx.x + blockDim.x * (threadIdx.y + blockDim.y * threadIdx.z);
@@ -732,10 +732,10 @@ __device__ __forceinline__ void stage_links(cx<typename P::R>* __restrict__ lu,
     const int nbr = e & 0x7fffffff;
     const bool neg = e < 0;
     const int site = d >= 4 ? nbr : i;
-    lnb[w] = nbr;
+    lnb[w - dst_base] = nbr;
     const int par = d >= 4 ? 1 - p : p;  // U_mu(x - mu) lives on the other parity
     const typename P::S* src = U + (size_t)((mu * 2 + par) * NE) * g.vh + site;
-    cx<R>* dst = lu + (size_t)w * 9;
+    cx<R>* dst = lu + (size_t)(w - dst_base) * 9;
     const float ls = link_scale<P>();
 #pragma unroll
     for (int e = 0; e < NE; ++e) dst[e] = Pack<P>::unpack(ld_link_elem<P>(src + (size_t)e * g.vh), ls);

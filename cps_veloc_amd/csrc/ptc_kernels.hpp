@@ -149,16 +149,19 @@ k_ptc_loss(uint32_t* __restrict__ gact, const uint32_t* __restrict__ a3,
 // activations (f_b = (Re f_b, Im f_b)):
 //   Re (M f)_a = sum_b dot2(R_ab, f_b),   Im (M f)_a = sum_b dot2(I_ab, f_b).
 // Each bf16 x bf16 product is exact in FP32.  The thread that staged link w
-// (stage_links: one thread per link of the tile) rewrites its 9 complex
-// entries in place, 18 words in the same 72 bytes (only_dir >= 0: only the
-// links of that staged direction).
+// (stage_links: thread w stages link w of the tile, w = direction * NT +
+// site) rewrites its 9 complex entries in place, 18 words in the same 72
+// bytes, so no barrier is needed in between (only_dir >= 0: only the NT
+// links of that direction, staged compactly from entry 0, so entry
+// threadIdx.x - only_dir * NT).
 __device__ __forceinline__ void pack_links_dot2(cx<float>* lu, int only_dir = -1) {
-  const int w = threadIdx.x;
-  if (w >= 8 * NT || (only_dir >= 0 && w / NT != only_dir)) return;
+  const int w = (int)threadIdx.x - (only_dir >= 0 ? only_dir * NT : 0);
+  if (w < 0 || w >= (only_dir >= 0 ? NT : 8 * NT)) return;
   cx<float> u[9];
 #pragma unroll
   for (int e = 0; e < 9; ++e) u[e] = lu[w * 9 + e];
-  const bool dag = w >= 4 * NT;  // staged direction w / NT = 4..7: backward links
+  // staged direction 4..7: backward links
+  const bool dag = only_dir >= 0 ? only_dir >= 4 : w >= 4 * NT;
   uint32_t* q = reinterpret_cast<uint32_t*>(lu + w * 9);
 #pragma unroll
   for (int a = 0; a < 3; ++a)
@@ -218,10 +221,12 @@ __device__ __forceinline__ void transport4(const uint32_t* lw, int d, const uint
 }
 
 // site of the activation neighbour of tile site n along d (S index)
+// (lslot: the slot of the staged neighbour list holding direction d, d - 1
+// when all eight directions are staged)
 __device__ __forceinline__ size_t nbr_site(const int* lnb, int d, int n, int par, int i0,
-                                           const Geom& g) {
+                                           const Geom& g, int lslot) {
   if (d == 0) return (size_t)par * g.vh + i0 + n;
-  return (size_t)(1 - par) * g.vh + lnb[(d - 1) * NT + n];
+  return (size_t)(1 - par) * g.vh + lnb[lslot * NT + n];
 }
 
 // (site, channel) items of a tile spread over the work-group: PER per thread
@@ -237,13 +242,15 @@ struct Items {
 template <int CIN>
 __device__ __forceinline__ void gather_items(u32x4 (&raw)[Items<CIN>::PER][3],
                                              const uint32_t* __restrict__ in, const int* lnb,
-                                             int d, int par, int i0, const Geom& g) {
+                                             int d, int par, int i0, const Geom& g,
+                                             int lslot = -1) {
+  if (lslot < 0) lslot = d - 1;
 #pragma unroll
   for (int j = 0; j < Items<CIN>::PER; ++j) {
     const int w = threadIdx.x + j * NTHR;
     if (Items<CIN>::EXACT || w < Items<CIN>::N) {
       const int n = w / CIN, i = w - n * CIN;
-      const size_t y = nbr_site(lnb, d, n, par, i0, g);
+      const size_t y = nbr_site(lnb, d, n, par, i0, g, lslot);
       const u32x4* src = reinterpret_cast<const u32x4*>(in + (y * CIN + i) * 12);
       raw[j][0] = src[0];
       raw[j][1] = src[1];
@@ -261,8 +268,8 @@ __device__ __forceinline__ void gather_items(u32x4 (&raw)[Items<CIN>::PER][3],
 template <int CIN, int RS = 8 * CIN + PADK>
 __device__ __forceinline__ void store_transported(uint16_t* sB,
                                                   const u32x4 (&raw)[Items<CIN>::PER][3],
-                                                  const uint32_t* lw, int d) {
-  constexpr int KP = 8 * CIN;
+                                                  const uint32_t* lw, int d, int lslot = -1) {
+  if (lslot < 0) lslot = d > 0 ? d - 1 : 0;
 #pragma unroll
   for (int j = 0; j < Items<CIN>::PER; ++j) {
     const int w = threadIdx.x + j * NTHR;
@@ -272,7 +279,7 @@ __device__ __forceinline__ void store_transported(uint16_t* sB,
                             raw[j][1][0], raw[j][1][1], raw[j][1][2], raw[j][1][3],
                             raw[j][2][0], raw[j][2][1], raw[j][2][2], raw[j][2][3]};
     uint32_t re[3][2], im[3][2];
-    transport4(lw + ((d > 0 ? d - 1 : 0) * NT + n) * 18, d, f, re, im);
+    transport4(lw + (lslot * NT + n) * 18, d, f, re, im);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       const int col = n * 3 + a;
@@ -456,11 +463,21 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
   }
 }
 
+// Weight gradient LDS: the [col][row] tiles of X^T and Y^T (row padding
+// PADW = 4 bf16 keeps three work-groups within a CU's 160 KB; the
+// transposing loads showed no bank-conflict cost against a padding of 8 or
+// 16) and the links and neighbours of the one direction it transports
+constexpr int PADW = 4;
+// work-groups per CU the weight-gradient kernel is built for: three (14 %
+// faster than two with the next tile's output gradient prefetched), except
+// the 12-channel shapes, whose registers do not fit three without spills
+template <int CIN, int COUT>
+constexpr int gw_occ() { return CIN != 12 && COUT != 12 ? 3 : 2; }
 template <int CIN, int COUT>
 constexpr size_t gw_lds() {
-  constexpr size_t x = (size_t)NCOL * (8 * COUT + PADK) * 2;
-  constexpr size_t y = (size_t)NCOL * (8 * CIN + PADK) * 2;
-  return x + y + (size_t)8 * NT * 9 * 8 + 8 * NT * 4;
+  constexpr size_t x = (size_t)NCOL * (8 * COUT + PADW) * 2;
+  constexpr size_t y = (size_t)NCOL * (8 * CIN + PADW) * 2;
+  return x + y + (size_t)NT * 9 * 8 + NT * 4;
 }
 
 // One MFMA operand fragment (8 bf16: rows r0 + l%16, columns c0 + 8(l/16) ..
@@ -508,12 +525,12 @@ __device__ __forceinline__ void load_x(u32x4 (&xv)[XTile<COUT>::PER][3],
 // t = blockIdx.x, blockIdx.x + gridDim.x, ...:
 // part[d][blockIdx.x][M'][K'] = sum X[M'][cols] Y_d[K'][cols]^T
 template <int CIN, int COUT>
-__global__ void __launch_bounds__(NTHR, 2)
+__global__ void __launch_bounds__(NTHR, (gw_occ<CIN, COUT>()))
 k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
          const uint32_t* __restrict__ in, const typename dwfk::P32::S* __restrict__ U, Geom g) {
   constexpr int MP = 8 * COUT, KP = 8 * CIN;
   constexpr int MB = MP / 16, KB2 = KP / 16, CB = NCOL / 32;  // contraction over columns
-  constexpr int XS = MP + PADK, YS = KP + PADK;  // row strides of the [col][row] tiles
+  constexpr int XS = MP + PADW, YS = KP + PADW;  // row strides of the [col][row] tiles
   static_assert(NCOL % 32 == 0 && (XS * 2) % 8 == 0 && (YS * 2) % 8 == 0, "tile shapes");
   // this wave's output blocks: rows [mb0, mb0 + RM) x columns [kb0, kb0 + RK)
   constexpr int RM = (MB + 1) / 2, RK = (KB2 + 1) / 2;
@@ -521,7 +538,7 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
   uint16_t* sX = reinterpret_cast<uint16_t*>(smem);  // X^T: [col][M']
   uint16_t* sY = sX + NCOL * XS;                     // Y^T: [col][K']
   cx<float>* lu = reinterpret_cast<cx<float>*>(smem + (size_t)NCOL * (XS + YS) * 2);
-  int* lnb = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(lu) + (size_t)8 * NT * 9 * 8);
+  int* lnb = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(lu) + (size_t)NT * 9 * 8);
   const int d = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tiles_per_par = g.vh / NT, ntiles = 2 * tiles_per_par;
@@ -532,17 +549,21 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
 #pragma unroll
     for (int j = 0; j < RK; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   // X of the next tile is loaded into registers before this tile's products
+  // (PFX), or at the start of the tile when registers are short
+  constexpr bool PFX = gw_occ<CIN, COUT>() == 2;
   u32x4 xv[XTile<COUT>::PER][3];
-  if (blockIdx.x < ntiles)
+  if (PFX && blockIdx.x < ntiles)
     load_x<COUT>(xv, gout, blockIdx.x / tiles_per_par,
                  (blockIdx.x - blockIdx.x / tiles_per_par * tiles_per_par) * NT, g);
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int par = t / tiles_per_par;
     const int i0 = (t - par * tiles_per_par) * NT;
+    if (!PFX) load_x<COUT>(xv, gout, par, i0, g);
     __syncthreads();  // previous tile's operands consumed
-    // only direction d's links and neighbours are read (none for d = 0)
+    // only direction d's links and neighbours are read (none for d = 0),
+    // staged compactly in slot 0
     if (d > 0) {
-      dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par, d - 1);
+      dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par, d - 1, (d - 1) * NT);
       pack_links_dot2(lu, d - 1);
     }
     // X^T of the tile: item (site n, channel o) holds words spin*3 + colour;
@@ -568,13 +589,14 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
     __syncthreads();  // links staged
     {
       u32x4 raw[Items<CIN>::PER][3];
-      gather_items<CIN>(raw, in, lnb, d, par, i0, g);
-      store_transported<CIN, YS>(sY, raw, reinterpret_cast<const uint32_t*>(lu), d);
+      gather_items<CIN>(raw, in, lnb, d, par, i0, g, 0);
+      store_transported<CIN, YS>(sY, raw, reinterpret_cast<const uint32_t*>(lu), d, 0);
     }
     __syncthreads();
     {
       const int tn = t + gridDim.x;
-      if (tn < ntiles) load_x<COUT>(xv, gout, tn / tiles_per_par, (tn - tn / tiles_per_par * tiles_per_par) * NT, g);
+      if (PFX && tn < ntiles)
+        load_x<COUT>(xv, gout, tn / tiles_per_par, (tn - tn / tiles_per_par * tiles_per_par) * NT, g);
     }
     __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the products
 #pragma unroll
