@@ -2484,7 +2484,7 @@ int eig_deflate(dwfgpu_s* h, EigWork& w, const DVec& e, const double* U, int def
 }
 
 // Flexible GCR(m) with the network as preconditioner (no CPS counterpart;
-// InvCg's stop rule and FP64 true residual judge it).  Per step: z = N(r),
+// InvCg's stop rule on the FP64 true residual decides when it is done).  Per step: z = N(r),
 // w = A z made orthogonal to the cycle's earlier w_j by classical
 // Gram-Schmidt with complex coefficients c_j = <w_j, w> / |w_j|^2 (the
 // network is not Hermitian), p = z updated with the same coefficients, then
