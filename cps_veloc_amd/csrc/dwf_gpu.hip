@@ -3942,11 +3942,13 @@ int dwfgpu_ptc_train(dwfgpu_t h, const dwfgpu_ptc_train_arg* a, double* loss_fir
     int made = 0;
     if (!st && a->pairs_per_solve > 0)
       st = ptc_pairs(h, a->mass, b64, a->stride, a->pairs_per_solve, R, E, &made);
-    // pairs whose correction is a Gaussian vector e and residual r = A e:
-    // they carry the high modes the CG residuals lack
+    // pairs whose correction is a Gaussian vector e (drawn on the device) and
+    // residual r = A e: they carry the high modes the CG residuals lack
     for (int j = 0; j < a->random_pairs && !st; ++j) {
-      dwfsynth_gaussian(host.data(), h->n, a->seed + 104729ull * (sidx * 977 + j + 1));
-      st = Ops<P32>::to_device(h, E[made], host.data());
+      hipLaunchKernelGGL(ptck::k_gauss_f32, blas_grid(), blas_block(), 0, h->st,
+                         reinterpret_cast<float*>(E[made].d), h->n,
+                         (uint64_t)(a->seed + 104729ull * (sidx * 977 + j + 1)));
+      if (hipGetLastError() != hipSuccess) st = DWFGPU_HIP_ERROR;
       if (!st) st = Ops<P32>::mdagm(h, a->mass, R[made], E[made], nullptr);
       if (!st) ++made;
     }

@@ -64,6 +64,32 @@ __host__ __device__ __forceinline__ int slice_parity(int action, int s) {
   return action == 0 ? (s + 1) & 1 : 1;
 }
 
+// n unit Gaussians into an FP32 field (its reals in storage order): the
+// counter-based construction of the host generator (SplitMix64 finaliser on
+// a keyed counter, 53-bit uniforms, Box-Muller), evaluated with the device's
+// FP64 log / sqrt / sincos, so that training pairs need no host vectors.
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__global__ void __launch_bounds__(256) k_gauss_f32(float* __restrict__ out, size_t n,
+                                                   uint64_t seed) {
+  const uint64_t key = mix64(seed ^ 0xD1B54A32D192ED03ULL);
+  const size_t npair = (n + 1) / 2;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < npair;
+       k += (size_t)gridDim.x * blockDim.x) {
+    const double u1 = ((mix64(key + 2 * k) >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double u2 = ((mix64(key + 2 * k + 1) >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(6.283185307179586476925286766559 * u2, &sn, &cs);
+    out[2 * k] = (float)(r * cs);
+    if (2 * k + 1 < n) out[2 * k + 1] = (float)(r * sn);
+  }
+}
+
 // CG half vector (policy P, times `scale`) -> activations with C = ls
 // channels; the other parity of every channel is cleared
 template <class P>
