@@ -3933,12 +3933,14 @@ int dwfgpu_ptc_train(dwfgpu_t h, const dwfgpu_ptc_train_arg* a, double* loss_fir
     st = alloc_vec<P32>(h, &R[j]);
     if (!st) st = alloc_vec<P32>(h, &E[j]);
   }
-  std::vector<double> host(h->n);
+  std::vector<double> host(a->pairs_per_solve > 0 ? h->n : 0);
   double first = -1.0, last = 0.0;
   int used = 0;
   for (int sidx = 0; sidx < a->nsolves && !st; ++sidx) {
-    dwfsynth_gaussian(host.data(), h->n, a->seed + 7919ull * sidx);
-    st = Ops<P64>::to_device(h, b64, host.data());
+    if (a->pairs_per_solve > 0) {  // the source of the solve whose iterates give pairs
+      dwfsynth_gaussian(host.data(), h->n, a->seed + 7919ull * sidx);
+      st = Ops<P64>::to_device(h, b64, host.data());
+    }
     int made = 0;
     if (!st && a->pairs_per_solve > 0)
       st = ptc_pairs(h, a->mass, b64, a->stride, a->pairs_per_solve, R, E, &made);
