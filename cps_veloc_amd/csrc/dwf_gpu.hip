@@ -1261,7 +1261,9 @@ int ptc_fwd_t(dwfgpu_s* h, uint32_t* out, const uint32_t* in, const uint16_t* A)
 // Work-groups per direction of the weight-gradient kernel: as many as fill
 // every CU once (all NDIR directions resident together), so the loop over
 // tiles ends at the same time everywhere (576 work-groups over 512 resident
-// slots had left a second round of 64 with the device almost idle).
+// slots had left a second round of 64 with the device almost idle), rounded
+// down to a multiple of the XCD count so that each XCD holds all directions
+// of its work-groups.
 template <int CIN, int COUT>
 int ptc_gw_grid(dwfgpu_s* h, int* nwg) {
   constexpr size_t lds = ptck::gw_lds<CIN, COUT>();
@@ -1269,8 +1271,10 @@ int ptc_gw_grid(dwfgpu_s* h, int* nwg) {
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->dev));
   HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ptck::k_ptc_gw<CIN, COUT>,
                                                       ptck::NTHR, lds));
-  *nwg = std::max(1, std::min(ncu * std::max(per_cu, 1) / ptck::NDIR,
-                              (int)(2 * h->g.vh / ptck::NT)));
+  int n = std::max(1, std::min(ncu * std::max(per_cu, 1) / ptck::NDIR,
+                               (int)(2 * h->g.vh / ptck::NT)));
+  if (n >= h->g.nxcd) n -= n % h->g.nxcd;
+  *nwg = n;
   return DWFGPU_OK;
 }
 
@@ -1281,8 +1285,8 @@ int ptc_gw_t(dwfgpu_s* h, float* part, int nwg, const uint32_t* gout, const uint
     set_err("ptc: gradient %dx%d needs %zu bytes of LDS", CIN, COUT, lds);
     return DWFGPU_BAD_ARG;
   }
-  hipLaunchKernelGGL((ptck::k_ptc_gw<CIN, COUT>), dim3(nwg, ptck::NDIR), dim3(ptck::NTHR), lds,
-                     h->st, part, gout, in, (const typename P32::S*)h->f[DWFGPU_F32].U, h->g);
+  hipLaunchKernelGGL((ptck::k_ptc_gw<CIN, COUT>), dim3(ptck::NDIR * nwg), dim3(ptck::NTHR), lds,
+                     h->st, part, nwg, gout, in, (const typename P32::S*)h->f[DWFGPU_F32].U, h->g);
   HIPCHK(hipGetLastError());
   return DWFGPU_OK;
 }

@@ -402,9 +402,11 @@ k_ptc_fwd(uint32_t* __restrict__ out, const uint32_t* __restrict__ in,
   cx<float>* lu = reinterpret_cast<cx<float>*>(smem + big);
   int* lnb = reinterpret_cast<int*>(smem + big + (size_t)8 * NT * 9 * 8);
 
-  const int tiles_per_par = g.vh / NT;
-  const int par = blockIdx.x / tiles_per_par;
-  const int i0 = (blockIdx.x - par * tiles_per_par) * NT;
+  // tile: each XCD sweeps a contiguous range, so neighbouring tiles share
+  // its L2
+  const int tiles_per_par = g.vh / NT, tile = dwfk::xcd_block(g.nxcd);
+  const int par = tile / tiles_per_par;
+  const int i0 = (tile - par * tiles_per_par) * NT;
   dwfk::stage_links<dwfk::P32>(lu, lnb, U, g, i0 / NT, NT, 1, par);
   pack_links_dot2(lu);
   const uint32_t* lw = reinterpret_cast<const uint32_t*>(lu);
@@ -547,12 +549,16 @@ __device__ __forceinline__ void load_x(u32x4 (&xv)[XTile<COUT>::PER][3],
   }
 }
 
-// Weight gradient partials: for direction d = blockIdx.y and the tiles
-// t = blockIdx.x, blockIdx.x + gridDim.x, ...:
-// part[d][blockIdx.x][M'][K'] = sum X[M'][cols] Y_d[K'][cols]^T
+// Weight gradient partials part[d][w][M'][K'] = sum X[M'][cols] Y_d[K'][cols]^T
+// over the tiles of work-group w of direction d.  The nine work-groups of
+// one w run on one XCD at the same time (work-group L goes to XCD L % nxcd),
+// so the output-gradient tile and the neighbour fields they share are read
+// into that XCD's L2 once; each XCD sweeps its own contiguous range of
+// tiles, nwg / nxcd of them at a time.  A grid of 9 nwg work-groups; with
+// nwg not a multiple of nxcd, one range over all tiles.
 template <int CIN, int COUT>
 __global__ void __launch_bounds__(NTHR, (gw_occ<CIN, COUT>()))
-k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
+k_ptc_gw(float* __restrict__ part, int nwg, const uint32_t* __restrict__ gout,
          const uint32_t* __restrict__ in, const typename dwfk::P32::S* __restrict__ U, Geom g) {
   constexpr int MP = 8 * COUT, KP = 8 * CIN;
   constexpr int MB = MP / 16, KB2 = KP / 16, CB = NCOL / 32;  // contraction over columns
@@ -565,9 +571,13 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
   uint16_t* sY = sX + NCOL * XS;                     // Y^T: [col][K']
   cx<float>* lu = reinterpret_cast<cx<float>*>(smem + (size_t)NCOL * (XS + YS) * 2);
   int* lnb = reinterpret_cast<int*>(reinterpret_cast<unsigned char*>(lu) + (size_t)NT * 9 * 8);
-  const int d = blockIdx.y;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int tiles_per_par = g.vh / NT, ntiles = 2 * tiles_per_par;
+  const int nx = nwg % g.nxcd == 0 ? g.nxcd : 1, m = nwg / nx;
+  const int x = blockIdx.x % nx, slot = blockIdx.x / nx;
+  const int d = slot % NDIR, k = slot / NDIR;  // k < m
+  const int t_end = (int)((long)(x + 1) * ntiles / nx);
+  const int t_first = (int)((long)x * ntiles / nx) + k;
   const int mb0 = (wave & 1) * RM, kb0 = (wave >> 1) * RK;
   f32x4 acc[RM][RK];
 #pragma unroll
@@ -578,10 +588,10 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
   // (PFX), or at the start of the tile when registers are short
   constexpr bool PFX = gw_occ<CIN, COUT>() == 2;
   u32x4 xv[XTile<COUT>::PER][3];
-  if (PFX && blockIdx.x < ntiles)
-    load_x<COUT>(xv, gout, blockIdx.x / tiles_per_par,
-                 (blockIdx.x - blockIdx.x / tiles_per_par * tiles_per_par) * NT, g);
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  if (PFX && t_first < t_end)
+    load_x<COUT>(xv, gout, t_first / tiles_per_par,
+                 (t_first - t_first / tiles_per_par * tiles_per_par) * NT, g);
+  for (int t = t_first; t < t_end; t += m) {
     const int par = t / tiles_per_par;
     const int i0 = (t - par * tiles_per_par) * NT;
     if (!PFX) load_x<COUT>(xv, gout, par, i0, g);
@@ -620,8 +630,8 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
     }
     __syncthreads();
     {
-      const int tn = t + gridDim.x;
-      if (PFX && tn < ntiles)
+      const int tn = t + m;
+      if (PFX && tn < t_end)
         load_x<COUT>(xv, gout, tn / tiles_per_par, (tn - tn / tiles_per_par * tiles_per_par) * NT, g);
     }
     __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the products
@@ -643,7 +653,7 @@ k_ptc_gw(float* __restrict__ part, const uint32_t* __restrict__ gout,
       }
     }
   }
-  float* dst = part + ((size_t)d * gridDim.x + blockIdx.x) * MP * KP;
+  float* dst = part + ((size_t)d * nwg + x * m + k) * MP * KP;
 #pragma unroll
   for (int i = 0; i < RM; ++i) {
     if (mb0 + i < MB) {
