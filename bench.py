@@ -173,11 +173,24 @@ def mrhs_full_sources(wl, seed=99):
     return firsts, seconds, np.array(masses)
 
 
-def run_batched(wl, sloppy, delta, max_iter):
+def bits_digest(x):
+    """XOR and wrapping sum of the 64-bit words of an FP64 array: equal for
+    bit-identical arrays (a cheap check, not a hash)"""
+    w = np.ascontiguousarray(x, dtype=np.float64).view(np.uint64)
+    return int(np.bitwise_xor.reduce(w)), int(np.sum(w, dtype=np.uint64))
+
+
+def run_batched(wl, sloppy, delta, max_iter, trace=False):
     """DiracOp*::MatInv over all RHS of the workload: one batched solve
     (dwfgpu_matinv_mrhs_dev: sources prepared and solutions completed on the
     device, one graph-replayed CG for all RHS) against the same entry point
-    called one RHS at a time.  Returns the record of the bench line."""
+    called one RHS at a time, and the batched solve again with plain kernel
+    launches instead of graph replays (what a single persistent launch could
+    at most save is the difference between launch-bound and GPU-bound).
+    trace: every call with plain launches (a kernel trace then covers them)
+    and 1 s with the GPU idle on either side of the timed batched call, so
+    that its kernels stand apart in a trace.  Returns the record of the
+    bench line."""
     from cps_veloc_amd import lattice as lat
     from cps_veloc_amd.dwf import CgArg, DiracOpDwfGpu, DiracOpMobiusGpu
     dims, ls = wl["dims"], wl["ls"]
@@ -193,35 +206,53 @@ def run_batched(wl, sloppy, delta, max_iter):
     arg = CgArg(0.0, max_iter, 1e-10)
     o1 = [op.multi_vector(1) for _ in range(4)]  # out_first, out_second, in_first, in_second
     # warm-up: autotuner, graph set-up and first-use allocations of both calls
-    op.MatInvMrhs_dev(outf, outs, inf, ins, CgArg(0.0, 3, 1e-10), masses, sloppy, delta)
+    op.MatInvMrhs_dev(outf, outs, inf, ins, CgArg(0.0, 3, 1e-10), masses, sloppy, delta,
+                      eager=trace)
     o1[2].upload(0, firsts[0])
     o1[3].upload(0, seconds[0])
     op.MatInvMrhs_dev(o1[0], o1[1], o1[2], o1[3], CgArg(0.0, 3, 1e-10), masses[:1], sloppy,
-                      delta)
+                      delta, eager=trace)
     for j in range(nrhs):
         outf.upload(j, np.zeros(op.n))
         outs.upload(j, np.zeros(op.n))
     log(f"[bench] batched MatInv of {nrhs} RHS")
+    if trace:
+        time.sleep(1.0)
     t0 = time.perf_counter()
-    it, tr, st, nrel = op.MatInvMrhs_dev(outf, outs, inf, ins, arg, masses, sloppy, delta)
+    it, tr, st, nrel = op.MatInvMrhs_dev(outf, outs, inf, ins, arg, masses, sloppy, delta,
+                                         eager=trace)
     t_batch = time.perf_counter() - t0
+    if trace:
+        time.sleep(1.0)
     ns, nr = op.mrhs_shape()
     log(f"[bench] batched: {t_batch:.2f}s, iters {it.tolist()}, shape ns={ns} nr={nr}")
-    t_seq, it_seq, err = 0.0, [], []
+
+    t_seq, it_seq, err, digests = 0.0, [], [], []
     for j in range(nrhs):
+        xb = np.concatenate([outf.download(j), outs.download(j)])
+        digests.append(bits_digest(xb))
         o1[2].upload(0, firsts[j])
         o1[3].upload(0, seconds[j])
         o1[0].upload(0, np.zeros(op.n))
         o1[1].upload(0, np.zeros(op.n))
         t0 = time.perf_counter()
         i1, _, _, _ = op.MatInvMrhs_dev(o1[0], o1[1], o1[2], o1[3], arg, masses[j:j + 1],
-                                        sloppy, delta)
+                                        sloppy, delta, eager=trace)
         t_seq += time.perf_counter() - t0
         it_seq.append(int(i1[0]))
         x1 = np.concatenate([o1[0].download(0), o1[1].download(0)])
-        xb = np.concatenate([outf.download(j), outs.download(j)])
         err.append(float(np.linalg.norm(xb - x1) / np.linalg.norm(x1)))
     log(f"[bench] one at a time: {t_seq:.2f}s, iters {it_seq}")
+    for j in range(nrhs):
+        outf.upload(j, np.zeros(op.n))
+        outs.upload(j, np.zeros(op.n))
+    t0 = time.perf_counter()
+    it_e, _, _, _ = op.MatInvMrhs_dev(outf, outs, inf, ins, arg, masses, sloppy, delta, eager=True)
+    t_eager = time.perf_counter() - t0
+    same = bool(np.array_equal(it_e, it)) and all(
+        bits_digest(np.concatenate([outf.download(j), outs.download(j)])) == digests[j]
+        for j in range(nrhs))
+    log(f"[bench] batched with plain launches: {t_eager:.2f}s (same solutions: {same})")
     for v in [inf, ins, outf, outs] + o1:
         v.free()
     op.close()
@@ -231,6 +262,8 @@ def run_batched(wl, sloppy, delta, max_iter):
             "batched": {"seconds": round(t_batch, 3), "iters": it.tolist(),
                         "true_res_max": float(np.max(tr)), "status": st.tolist(),
                         "reliable_updates": nrel.tolist()},
+            "batched_plain_launches": {"seconds": round(t_eager, 3),
+                                       "same_iters_and_solutions": same},
             "one_at_a_time": {"seconds": round(t_seq, 3), "iters": it_seq},
             "rhs_per_second_batched": round(nrhs / t_batch, 3),
             "speedup_batched_vs_one_at_a_time": round(t_seq / t_batch, 3),
@@ -239,7 +272,7 @@ def run_batched(wl, sloppy, delta, max_iter):
 
 def bench_mrhs(args, wl):
     """Bench line of the batched workload alone (--workload ..._mrhs24)."""
-    rec = run_batched(wl, args.mrhs_sloppy, args.delta, args.max_iter)
+    rec = run_batched(wl, args.mrhs_sloppy, args.delta, args.max_iter, args.mrhs_trace)
     out = {
         "metric": "batched 24-RHS MatInv to 1e-10: right-hand sides solved per second",
         "value": rec["rhs_per_second_batched"], "unit": "RHS/s", "n_gpus": 1, "steps": 1,
@@ -269,6 +302,9 @@ def main():
     ap.add_argument("--delta", type=float, default=0.1,
                     help="reliable update when |r| drops by this factor")
     ap.add_argument("--mrhs-sloppy", default="f32", choices=["f32", "bf16", "i16"])
+    ap.add_argument("--mrhs-trace", action="store_true",
+                    help="batched leg with plain launches only, the timed batched call "
+                         "between 1 s idle gaps (for a kernel trace)")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the 24-RHS batched MatInv leg (DWF 16^3x32, Ls 16)")
     ap.add_argument("--no-learned", action="store_true",
@@ -446,7 +482,7 @@ def main():
     if not args.no_batched:
         log("[bench] 24-RHS batched MatInv, DWF 16^3x32 Ls 16")
         batched = run_batched(WORKLOADS["dwf_16x16x16x32_ls16_mrhs24"], args.mrhs_sloppy,
-                              args.delta, args.max_iter)
+                              args.delta, args.max_iter, args.mrhs_trace)
 
     cpu, parity = None, None
     if not args.no_cpu:
