@@ -114,6 +114,32 @@ def test_network_gradients_match_oracle(action, ls):
         assert err < 6e-2
 
 
+def test_gradients_on_fewer_tiles_than_xcds():
+    """4x4x2x4: four 32-site tiles, so the weight gradient runs four
+    work-groups per direction in one range (fewer than the XCDs) and the
+    forward layer four work-groups over eight XCDs."""
+    dims, ls = (4, 4, 2, 4), 4
+    say(f"[gpu] network gradients on {dims} ls={ls}")
+    U, g = make("mobius", dims, ls)
+    g.ptc_init(1, 0.02)
+    rng = np.random.default_rng(5)
+    Ws = rounded_weights(g, rng, ls, scale=0.2)
+    vh = int(np.prod(dims)) // 2
+    r = lat.to_reals(lat.gaussian_spinor((ls, vh), 13))
+    e = lat.to_reals(lat.gaussian_spinor((ls, vh), 14))
+    links = pn.links_with_bc(U, dims)
+    err = rel(g.ptc_apply(r), pn.apply_net(Ws, r, links, dims, ls, "mobius"))
+    say(f"  forward normwise rel err {err:.3e}")
+    assert err < 3e-2
+    loss, grad = g.ptc_loss_grad(r, e)
+    wl, wg = pn.loss_and_grads_cg(Ws, r, e, links, dims, ls, "mobius")
+    assert abs(loss - wl) < 3e-2 * wl
+    for k, (a, b) in enumerate(zip(split_layers(grad, ls), wg)):
+        err = rel(a, b)
+        say(f"  layer {k}: gradient normwise rel err {err:.3e}")
+        assert err < 6e-2
+
+
 def test_training_and_preconditioned_solve():
     dims, ls, mass = (8, 8, 8, 8), 8, 0.04
     say(f"[gpu] train the network and solve with it, dwf {dims} ls={ls} mf={mass}")
